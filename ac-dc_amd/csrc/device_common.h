@@ -1,0 +1,190 @@
+// device_common.h -- shared device/host helpers of the gfx950 PCDM kernels:
+// Philox4x32-10 counter-based generator, the keyed per-step permutation that
+// turns step s into tau distinct coordinates, and wave-level reductions.
+//
+// The reference samples with per-thread GSL mt19937 streams seeded by thread
+// id (cpp/src/utils/randomNumbersUtil.h:83-125, used at
+// cpp/src/solver/MulticoreEngineExecutor.h:86) and tau-distinct sets by
+// rejection (cpp/src/experiments/cdn_common.h:238-248).  Here coordinate i of
+// step s is perm_{key(seed,s)}(i): a pure function of (seed, s, i) that any
+// wave can evaluate independently and the host can replay bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace acdc {
+
+// records the text returned by acdc_last_error (defined in solver.hip)
+void set_error(const char *msg);
+
+#define ACDC_HD __host__ __device__ __forceinline__
+
+// Philox4x32-10, J. Salmon et al., "Parallel random numbers: as easy as
+// 1, 2, 3" (SC'11): multipliers and Weyl increments from the paper.
+struct Philox {
+  static constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  static constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+
+  ACDC_HD static void round_fn(uint32_t &c0, uint32_t &c1, uint32_t &c2,
+                               uint32_t &c3, uint32_t k0, uint32_t k1) {
+    const uint64_t a = (uint64_t)M0 * c0;
+    const uint64_t bb = (uint64_t)M1 * c2;
+    const uint32_t n0 = (uint32_t)(bb >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)bb;
+    c3 = (uint32_t)a;
+    c0 = n0;
+    c2 = n2;
+  }
+
+  ACDC_HD static void gen(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      round_fn(c[0], c[1], c[2], c[3], k0, k1);
+      k0 += W0;
+      k1 += W1;
+    }
+  }
+};
+
+// Tag that separates the coordinate stream from the data-generation streams.
+constexpr uint32_t kSamplerTag = 0x50434453u;  // "PCDS"
+
+struct StepKeys {
+  uint32_t k[4];
+};
+
+ACDC_HD StepKeys step_keys(uint64_t seed, uint64_t step) {
+  uint32_t c[4] = {(uint32_t)step, (uint32_t)(step >> 32), kSamplerTag, 0u};
+  Philox::gen(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  StepKeys s;
+  s.k[0] = c[0];
+  s.k[1] = c[1];
+  s.k[2] = c[2];
+  s.k[3] = c[3];
+  return s;
+}
+
+// Feistel round function: a 32-bit multiply/xor-shift integer mixer.
+ACDC_HD uint32_t mix32(uint32_t x, uint32_t k) {
+  uint32_t h = x ^ k;
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+
+// Keyed bijection of [0, n): a four-round alternating-xor Feistel network on
+// [0, 2^b), 2^b >= n, with cycle walking back into [0, n).  Each round xors a
+// function of one half into the other, so every round -- and the network --
+// is invertible.  The walk ends because the orbit of i < n returns to i.
+struct Perm {
+  uint32_t n, hb, ma, mb;
+
+  ACDC_HD static Perm make(uint32_t n) {
+    uint32_t b = 1;
+    while (b < 32 && ((uint64_t)1 << b) < (uint64_t)n) ++b;
+    Perm p;
+    p.n = n;
+    p.hb = b >> 1;
+    const uint32_t ha = b - p.hb;
+    p.mb = p.hb == 0 ? 0u : (uint32_t)(((uint64_t)1 << p.hb) - 1);
+    p.ma = (uint32_t)(((uint64_t)1 << ha) - 1);
+    return p;
+  }
+
+  ACDC_HD uint32_t operator()(uint32_t i, const StepKeys &key) const {
+    uint32_t v = i;
+    do {
+      uint32_t lo = v & mb;
+      uint32_t hi = hb >= 32 ? 0u : (v >> hb);
+      hi ^= mix32(lo, key.k[0]) & ma;
+      lo ^= mix32(hi, key.k[1]) & mb;
+      hi ^= mix32(lo, key.k[2]) & ma;
+      lo ^= mix32(hi, key.k[3]) & mb;
+      v = (hi << hb) | lo;
+    } while (v >= n);
+    return v;
+  }
+};
+
+// ---------------------------------------------------------------- reductions
+// Sum over aligned groups of G lanes (G = 8..64) of one 64-wide wavefront;
+// every lane of the group receives the group total.  The xor butterfly has a
+// fixed order, so results are reproducible run to run.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+template <int G, typename T>
+__device__ __forceinline__ T group_max(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, G);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+template <int G, typename T>
+__device__ __forceinline__ T group_min(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, G);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------- coordinate steps
+// The three proximal rules.  g is the coordinate's partial derivative of the
+// smooth part evaluated at the step's state (the column.residual dot for the
+// square losses, the sample.w dot for the SVM), L the reciprocal ESO-scaled
+// Lipschitz constant.  Returns delta; *coef receives the factor that
+// multiplies the coordinate's stored values in the residual scatter.
+enum LossId : int { kLasso = 0, kSvm = 2, kNnls = 10 };
+
+struct StepScalars {
+  double lambda;     // Lasso lambda / SVM lambda
+  double lambda_n;   // SVM: lambda * n
+  double inv_lambda_n;
+};
+
+template <int LOSS>
+__device__ __forceinline__ double prox_delta(double g, double xj, double Lj,
+                                             double yj, const StepScalars &s,
+                                             double *coef) {
+  double d;
+  if constexpr (LOSS == kLasso) {
+    // sgn(v)(|v| - L lambda)_+ - x: MulticoreL1RegSquareLoss.h:41-51 with
+    // compute_soft_treshold, treshhold_functions.h:27-33
+    const double t = Lj * s.lambda;
+    const double v = xj - Lj * g;
+    double a = fabs(v) - t;
+    a = a < 0.0 ? 0.0 : a;
+    const double sg = (double)((v > 0.0) - (v < 0.0));
+    d = sg * a - xj;
+    *coef = d;
+  } else if constexpr (LOSS == kNnls) {
+    // projected coordinate step onto x >= 0 (no reference counterpart)
+    const double v = xj - Lj * g;
+    d = (v > 0.0 ? v : 0.0) - xj;
+    *coef = d;
+  } else {
+    // box-clipped dual step: MulticoreSVMDualLoss.h:119-141
+    d = (1.0 - g * yj) * s.lambda_n * Lj;
+    if (d < -xj)
+      d = -xj;
+    else if (d > 1.0 - xj)
+      d = 1.0 - xj;
+    *coef = d * yj * s.inv_lambda_n;
+  }
+  return d;
+}
+
+}  // namespace acdc

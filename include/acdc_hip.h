@@ -1,0 +1,195 @@
+/*
+ * acdc_hip.h -- C ABI of the MI355X (gfx950) PCDM hot path.
+ *
+ * Parallel randomized coordinate descent with ESO step scaling for
+ *   loss 0  (ACDC_LOSS_L1_SQUARE)      min 0.5||Ax-b||^2 + lambda||x||_1
+ *   loss 2  (ACDC_LOSS_SVM_DUAL_HINGE) hinge-SVM dual, box-clipped SDCA
+ *   loss 10 (ACDC_LOSS_NNLS)           min_{x>=0} 0.5||Bx-f||^2 (TTD)
+ * Loss ids follow the reference's settings->lossFunction switch
+ * (cpp/src/solver/MulticoreEngineExecutor.h:32-58); 10 is new.
+ *
+ * Conventions: every entry point returns an acdc_status; nothing throws
+ * across this boundary.  Host arrays belong to the caller and are copied at
+ * create; device memory belongs to the library.  Handles are independent (no
+ * globals besides the thread-local error text), calls on one handle must be
+ * serialised by the caller.  The residual sign convention is r = Ax - b
+ * (cpp/src/solver/lossFunctions/squareLoss/MulticoreSquareLoss.h:42-53).
+ * Sizes are int64_t, indices int32_t, values double.
+ */
+#ifndef ACDC_HIP_H
+#define ACDC_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum acdc_status {
+  ACDC_OK = 0,
+  ACDC_ERR_INVALID = 1,  /* bad argument or inconsistent sizes */
+  ACDC_ERR_HIP = 2,      /* HIP runtime error (text in acdc_last_error) */
+  ACDC_ERR_NOMEM = 3,    /* host or device allocation failed */
+  ACDC_ERR_STATE = 4,    /* call not valid for this handle (e.g. wrong loss) */
+  ACDC_ERR_NO_DEVICE = 5 /* no usable gfx950 device */
+} acdc_status;
+
+typedef enum acdc_loss {
+  ACDC_LOSS_L1_SQUARE = 0,
+  ACDC_LOSS_SVM_DUAL_HINGE = 2,
+  ACDC_LOSS_NNLS = 10
+} acdc_loss;
+
+typedef enum acdc_precision {
+  ACDC_PREC_FP64 = 0,        /* plain FP64 everywhere */
+  ACDC_PREC_FP32_REFRESH = 1 /* FP32 values/residual/dots/scatter; FP64 x,
+                                Lipschitz, periodic r = Ax-b refresh, gap */
+} acdc_precision;
+
+typedef enum acdc_beta_mode {
+  ACDC_BETA_OMEGA = 0,  /* beta = 1+(omega-1)(tau-1)/(n-1), omega = max nnz
+                           per row: MultiCoreSolver.cpp:32-38, :108-110 */
+  ACDC_BETA_FORCED = 1, /* beta = forced_beta (OptimizationSettings
+                           ::forcedSigma, settingsAndStatistics.h:53) */
+  ACDC_BETA_NONE = 2    /* beta = 1, the multicore Lasso's L_j = 1/||a_j||^2
+                           (MulticoreSquareLoss.h:27-40) */
+} acdc_beta_mode;
+
+typedef enum acdc_step_mode {
+  ACDC_STEP_SYNC = 0 /* tau distinct coordinates per step, all deltas from
+                        one state then applied (cdn_common.h:235-282);
+                        tau = 1 is the serial method (loss_square.h:72-83) */
+} acdc_step_mode;
+
+/* Solver settings; the reference counterpart is OptimizationSettings
+ * (cpp/src/solver/settingsAndStatistics.h:11-93) plus ProblemData::lambda
+ * and ::sigma (structures.h:30-97). */
+typedef struct acdc_settings {
+  double lambda;           /* ProblemData::lambda; SVM: lambda of the primal */
+  int64_t tau;             /* coordinates per step (totalThreads, minibatch b) */
+  uint64_t seed;           /* key of the Philox4x32-10 coordinate stream */
+  int32_t beta_mode;       /* acdc_beta_mode */
+  double forced_beta;      /* used when beta_mode == ACDC_BETA_FORCED */
+  int32_t precision;       /* acdc_precision */
+  int32_t step_mode;       /* acdc_step_mode */
+  int32_t lanes_per_coord; /* 0 = auto, else 8, 16, 32 or 64 lanes per
+                              sampled coordinate */
+  int32_t refresh_period;  /* FP32 path: FP64 residual refresh every this
+                              many epochs inside acdc_run_updates (0 = off) */
+} acdc_settings;
+
+/* One evaluation, all in FP64.  Counterparts: SquareLoss::getResultLogRow
+ * (squareLoss/SquareLoss.h:28-38) and SVMDualLoss::getResultLogRow
+ * (SVMDualLoss/SVMDualLoss.h:31-42). */
+typedef struct acdc_report {
+  double primal;    /* Lasso 0.5||r||^2+lambda||x||_1; NNLS 0.5||r||^2;
+                       SVM (lambda/2)||w||^2 + mean hinge */
+  double dual;      /* Lasso D(theta), theta = -r min(1,lambda/||A^T r||_inf);
+                       SVM -(lambda/2)||w||^2 + mean alpha; NNLS: primal - xg */
+  double gap;       /* primal - dual */
+  double rel_gap;   /* gap / |primal| */
+  double grad_inf;  /* ||A^T r||_inf (Lasso, NNLS); 0 for SVM */
+  double x_l1;      /* ||x||_1 (sum alpha for SVM) */
+  double res_sq;    /* ||r||^2 (||w||^2 for SVM) */
+  double kkt;       /* NNLS max_j |min(x_j, g_j)|; 0 otherwise */
+  double accuracy;  /* SVM training 0-1 accuracy; 0 otherwise */
+  double nnz_x;     /* number of nonzero coordinates */
+  uint64_t updates; /* coordinate updates applied since create */
+  double epochs;    /* updates / n */
+  double kernel_ms; /* device time of the update kernels so far */
+  double eval_ms;   /* device time of refresh + evaluation kernels so far */
+} acdc_report;
+
+typedef struct acdc_problem acdc_problem;
+typedef struct acdc_solver acdc_solver;
+
+/* Text of the last error on this host thread ("" if none). */
+const char *acdc_last_error(void);
+
+/* Number of visible gfx950 devices (0 when none). */
+acdc_status acdc_device_count(int32_t *count);
+
+/* Column-major data A (m x n) and b (m) for losses 0 and 10.
+ * Replaces ProblemData<L,D>::A_csc_* + b as filled by
+ * InputOuputHelper::loadCSCData (cpp/src/utils/file_reader.h:108-137).
+ * Row indices must lie in [0,m); duplicates within a column are allowed. */
+acdc_status acdc_problem_create_csc(int64_t m, int64_t n, int64_t nnz,
+                                    const int32_t *col_ptr,
+                                    const int32_t *row_idx, const double *vals,
+                                    const double *b, acdc_problem **out);
+
+/* Sample-major data for loss 2: n_samples rows of n_features, labels +-1.
+ * Replaces ProblemData::A_csr_* + b as filled by
+ * loadDistributedSparseSVMRowData (utils/distributed_instances_loader.h:68-193),
+ * where n = samples and m = features. */
+acdc_status acdc_problem_create_csr(int64_t n_samples, int64_t n_features,
+                                    int64_t nnz, const int32_t *row_ptr,
+                                    const int32_t *col_idx, const double *vals,
+                                    const double *labels, acdc_problem **out);
+
+acdc_status acdc_problem_destroy(acdc_problem *p);
+
+/* omega = max nnz per row (CSC) or max samples per feature (CSR):
+ * MultiCoreSolver.cpp:32-38 and :55-62. */
+acdc_status acdc_problem_omega(const acdc_problem *p, int32_t *omega);
+
+/* Builds the device data, sets x = 0, r = Ax - b (MulticoreEngineExecutor
+ * constructor, MulticoreEngineExecutor.h:25-66: x zeroed :29-30,
+ * recomputeResiduals :59, computeReciprocalLipConstants :60). */
+acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
+                               const acdc_settings *s, acdc_solver **out);
+acdc_status acdc_solver_destroy(acdc_solver *s);
+
+/* Runs ceil(n_updates / tau) steps of tau coordinates.
+ * Replaces AbstractEngineExecutor::executeBulkOfIterations
+ * (MulticoreEngineExecutor.h:83-89) with innerIterations = n_updates. */
+acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates);
+
+/* r = Ax - b in FP64 (SVM: w = (1/(lambda n)) sum alpha_i y_i x_i).
+ * Replaces AbstractLoss::recomputeResiduals (MulticoreSquareLoss.h:42-53,
+ * MulticoreSVMDualLoss.h:31-46). */
+acdc_status acdc_recompute_residuals(acdc_solver *s);
+
+/* FP64 primal/dual/gap.  Replaces AbstractLoss::computeObjectiveValue
+ * (MulticoreL1RegSquareLoss.h:30-39, MulticoreSVMDualLoss.h:63-117).
+ * On the FP32 path the residual is first recomputed in FP64 (and written
+ * back to the FP32 residual); on the FP64 path the running residual is used. */
+acdc_status acdc_report_get(acdc_solver *s, acdc_report *out);
+
+/* x (alpha for SVM), length n.  acdc_set_x is the warm start of
+ * lasso_console_solver.cpp:153-172 and recomputes the residual. */
+acdc_status acdc_get_x(acdc_solver *s, double *x, int64_t n);
+acdc_status acdc_set_x(acdc_solver *s, const double *x, int64_t n);
+
+/* Running residual r (Lasso/NNLS, length m) or w (SVM, n_features). */
+acdc_status acdc_get_residuals(acdc_solver *s, double *r, int64_t len);
+
+/* Reciprocal Lipschitz constants L_j = 1/(beta ||a_j||^2) (ProblemData::Li). */
+acdc_status acdc_get_lipschitz(acdc_solver *s, double *Li, int64_t n);
+
+/* lambda for the next updates (lambda path, lasso_console_solver.cpp:149-161);
+ * SVM: also rescales w for the new 1/(lambda n). */
+acdc_status acdc_set_lambda(acdc_solver *s, double lambda);
+
+/* ESO beta in use (ProblemData::sigma, MultiCoreSolver.cpp:108-110). */
+acdc_status acdc_get_beta(const acdc_solver *s, double *beta);
+
+/* Coordinates the device samples for steps [step0, step0+nsteps): out has
+ * nsteps*tau entries, row-major.  Used to replay the stream on the host. */
+acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0,
+                                    uint64_t nsteps, int32_t *out);
+
+/* Runs nsteps (1..4096) synchronous steps (tau >= 2) with HIP events around
+ * each kernel and returns the mean device time per launch of the gather
+ * (phase A) and scatter (phase B) kernels in milliseconds.  The steps count
+ * as regular updates.  Counterpart of the reference's per-launch cudaEvent
+ * timing (oldSourceFiles/cuda_hso/hso/solver/parallel/randomL2L1Solver.h:169-183). */
+acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps,
+                                      double *gather_ms, double *scatter_ms);
+
+/* Waits for all work queued on the solver's stream. */
+acdc_status acdc_synchronize(acdc_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ACDC_HIP_H */
