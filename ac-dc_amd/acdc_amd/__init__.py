@@ -33,6 +33,9 @@ BETA_OMEGA = 0
 BETA_FORCED = 1
 BETA_NONE = 2
 STEP_SYNC = 0
+LAUNCH_AUTO = 0
+LAUNCH_PER_STEP = 1
+LAUNCH_PERSISTENT = 2
 
 NNZ_FIXED = 0
 NNZ_ZIPF_TRUNC = 1
@@ -55,7 +58,7 @@ class Settings(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("beta_mode", ctypes.c_int32),
                 ("forced_beta", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("step_mode", ctypes.c_int32), ("lanes_per_coord", ctypes.c_int32),
-                ("refresh_period", ctypes.c_int32)]
+                ("refresh_period", ctypes.c_int32), ("launch_mode", ctypes.c_int32)]
 
 
 class _Report(ctypes.Structure):
@@ -282,13 +285,14 @@ class Solver:
     def __init__(self, problem: Problem, loss: int, *, lam: float, tau: int = 1, seed: int = 1,
                  beta_mode: int = BETA_OMEGA, forced_beta: float = 1.0,
                  precision: int = PREC_FP64, step_mode: int = STEP_SYNC,
-                 lanes_per_coord: int = 0, refresh_period: int = 0):
+                 lanes_per_coord: int = 0, refresh_period: int = 0,
+                 launch_mode: int = LAUNCH_AUTO):
         self.problem = problem
         self.loss = loss
         self.n = problem.n
         self.res_len = problem.m
         st = Settings(lam, tau, seed, beta_mode, forced_beta, precision, step_mode,
-                      lanes_per_coord, refresh_period)
+                      lanes_per_coord, refresh_period, launch_mode)
         self.settings = st
         h = _P()
         _check(lib.acdc_solver_create(problem._h, loss, ctypes.byref(st), ctypes.byref(h)))

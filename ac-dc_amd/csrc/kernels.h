@@ -3,13 +3,20 @@
 //
 // Data are "segments": the coordinate-major matrix holds one segment per
 // coordinate (a CSC column for Lasso/NNLS, a CSR sample row for the SVM
-// dual) and the residual-major matrix one segment per residual entry (a CSR
-// row of A; a feature column of the SVM data).  A group of G lanes (G = 8,
-// 16, 32 or 64, aligned inside the 64-wide wavefront) owns one segment: its
-// lanes stride over the segment with unit stride, so the index and value
-// streams are read coalesced, and the partial sums meet in a fixed xor
-// butterfly.  MFMA is not used: every kernel here is a sparse gather/scatter
-// bound by HBM and L2 traffic.
+// dual).  A group of G lanes (G = 8, 16, 32 or 64, aligned inside the
+// 64-wide wavefront) owns one segment: its lanes stride over the segment with
+// unit stride, so the index and value streams are read coalesced, and the
+// partial sums meet in a fixed xor butterfly.  MFMA is not used: every kernel
+// here is a sparse gather/scatter bound by HBM and L2 traffic.
+//
+// Residual representation.  FP64 path: res is the running residual r (the
+// SVM's w) in FP64.  FP32 path: the residual is anchored,
+//     r = r64 + dr,   g_j = gref_j + <a_j, dr>,
+// where r64 = Ax - b and gref = A^T r64 are recomputed in FP64 -- from the
+// FP64 values of A -- at every refresh, and dr (FP32) accumulates only the
+// updates since then.  The FP32 terms are corrections whose rounding error
+// shrinks with the step sizes, so the iteration converges to the solution of
+// the FP64 problem (iterative refinement), not of the FP32-rounded one.
 #pragma once
 #include "device_common.h"
 
@@ -22,7 +29,32 @@ struct Seg {
   const T *__restrict__ val;        // nnz values
 };
 
-constexpr int kBlock = 256;  // threads per block of the grid-wide kernels
+constexpr int kBlock = 256;  // threads per block
+
+template <typename T>
+struct StepArgs {
+  Seg<T> A;
+  T *res;              // FP64: running residual; FP32: correction dr
+  const double *gref;  // FP32: anchored gradient A^T r64; FP64: null
+  double *x;
+  const double *L;
+  const double *y;     // SVM labels (null otherwise)
+  StepScalars sc;
+  Perm perm;
+  uint64_t seed;
+  int64_t tau;
+  int32_t *coord;
+  double *coef;
+};
+
+// One coordinate's proximal step from the group's reduced partial sum; every
+// lane of the group evaluates it identically.
+template <typename T, int LOSS>
+__device__ __forceinline__ double coord_delta(const StepArgs<T> &a, uint32_t j,
+                                              double acc, double xj, double *c) {
+  const double g = a.gref ? a.gref[j] + acc : acc;
+  return prox_delta<LOSS>(g, xj, a.L[j], LOSS == kSvm ? a.y[j] : 0.0, a.sc, c);
+}
 
 // ------------------------------------------------------------------ phase A
 // Synchronous step s, gather half: group -> slot i < tau, coordinate
@@ -33,29 +65,23 @@ constexpr int kBlock = 256;  // threads per block of the grid-wide kernels
 // (MulticoreL1RegSquareLoss.h:41-56, MulticoreSVMDualLoss.h:119-134) with the
 // synchronous semantics of cdn_common.h:255-270.
 template <typename T, int G, int LOSS>
-__global__ __launch_bounds__(kBlock) void k_step_gather(
-    Seg<T> A, const T *__restrict__ res, double *__restrict__ x,
-    const double *__restrict__ L, const double *__restrict__ y, StepScalars sc,
-    Perm perm, uint64_t seed, uint64_t step, int64_t tau,
-    int32_t *__restrict__ coord, double *__restrict__ coef) {
+__global__ __launch_bounds__(kBlock) void k_step_gather(StepArgs<T> a, uint64_t step) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t slot = t / G;
   const int lane = threadIdx.x & (G - 1);
-  if (slot >= tau) return;  // whole groups leave together
-  const StepKeys key = step_keys(seed, step);
-  const uint32_t j = perm((uint32_t)slot, key);
-  const int32_t p0 = A.ptr[j], p1 = A.ptr[j + 1];
+  if (slot >= a.tau) return;  // whole groups leave together
+  const uint32_t j = a.perm((uint32_t)slot, step_keys(a.seed, step));
+  const int32_t p0 = a.A.ptr[j], p1 = a.A.ptr[j + 1];
   T acc = 0;
-  for (int32_t p = p0 + lane; p < p1; p += G) acc += A.val[p] * res[A.idx[p]];
+  for (int32_t p = p0 + lane; p < p1; p += G) acc += a.A.val[p] * a.res[a.A.idx[p]];
   acc = group_sum<G>(acc);
+  const double xj = a.x[j];
+  double c;
+  const double d = coord_delta<T, LOSS>(a, j, (double)acc, xj, &c);
   if (lane == 0) {
-    const double xj = x[j];
-    double c;
-    const double d =
-        prox_delta<LOSS>((double)acc, xj, L[j], LOSS == kSvm ? y[j] : 0.0, sc, &c);
-    x[j] = xj + d;
-    coord[slot] = (int32_t)j;
-    coef[slot] = c;
+    a.x[j] = xj + d;
+    a.coord[slot] = (int32_t)j;
+    a.coef[slot] = c;
   }
 }
 
@@ -65,20 +91,173 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(
 // parallel_essentials_posix.h:62-78).  A zero coefficient -- the common case
 // for Lasso coordinates that stay at zero -- skips the segment entirely.
 template <typename T, int G>
-__global__ __launch_bounds__(kBlock) void k_step_scatter(
-    Seg<T> A, T *__restrict__ res, const int32_t *__restrict__ coord,
-    const double *__restrict__ coef, int64_t tau) {
+__global__ __launch_bounds__(kBlock) void k_step_scatter(StepArgs<T> a) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t slot = t / G;
   const int lane = threadIdx.x & (G - 1);
-  if (slot >= tau) return;
-  const double c = coef[slot];
+  if (slot >= a.tau) return;
+  const double c = a.coef[slot];
   if (c == 0.0) return;
-  const int32_t j = coord[slot];
+  const int32_t j = a.coord[slot];
   const T ct = (T)c;
-  const int32_t p0 = A.ptr[j], p1 = A.ptr[j + 1];
-  for (int32_t p = p0 + lane; p < p1; p += G)
-    unsafeAtomicAdd(&res[A.idx[p]], ct * A.val[p]);
+  const int32_t p0 = a.A.ptr[j], p1 = a.A.ptr[j + 1];
+  for (int32_t p = p0 + lane; p < p1; p += G) unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
+}
+
+// ------------------------------------------------------ grid-wide barrier
+// Sense-counting barrier over all blocks of a cooperative launch.  Arrival is
+// one agent-scope atomic per block; the last block bumps the generation.  The
+// wait is bounded: after spin_limit polls the block raises the shared error
+// flag, every waiting block sees it and returns false, and the kernel ends.
+//
+// No cache-wide fence is issued (an agent-scope fence writes back and
+// invalidates the whole L2 of the XCD, once per block).  Instead every access
+// that crosses blocks inside the persistent kernel is itself an agent-scope
+// atomic (residual gathers and scatters, x_j), and each wave waits for its own
+// memory operations to complete before the block arrives, so an arrival is
+// only counted once the block's atomics have been performed.
+// Two levels: block b arrives on sub-counter b % kSubBarriers; the last block
+// of each subgroup arrives on the top counter; the last subgroup publishes the
+// new generation to one release word per subgroup, which only that
+// subgroup's blocks poll.  Every counter and word sits on its own 256-byte
+// line, so arrivals are spread over kSubBarriers addresses and polling never
+// contends with arrivals.
+constexpr int kSubBarriers = 16;
+
+struct alignas(256) SyncWord {
+  unsigned int v;
+  unsigned int pad[63];
+};
+
+struct GridSync {
+  SyncWord top;
+  SyncWord error;
+  SyncWord sub[kSubBarriers];
+  SyncWord release[kSubBarriers];
+};
+
+__device__ __forceinline__ void wait_memory() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+__device__ __forceinline__ unsigned int atomic_inc_agent(unsigned int *p) {
+  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned int load_agent(const unsigned int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(unsigned int *p, unsigned int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool grid_sync(GridSync *gs, unsigned int &gen,
+                                          unsigned int spin_limit) {
+  __shared__ unsigned int s_ok;
+  wait_memory();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int want = gen + 1;
+    const unsigned int nb = gridDim.x;
+    const unsigned int nsub = nb < (unsigned)kSubBarriers ? nb : (unsigned)kSubBarriers;
+    const unsigned int sub = blockIdx.x % nsub;
+    const unsigned int in_sub = (nb - sub + nsub - 1) / nsub;  // blocks b with b % nsub == sub
+    bool released = false;
+    if (atomic_inc_agent(&gs->sub[sub].v) == in_sub - 1) {
+      store_agent(&gs->sub[sub].v, 0u);
+      if (atomic_inc_agent(&gs->top.v) == nsub - 1) {
+        store_agent(&gs->top.v, 0u);
+        wait_memory();
+        for (unsigned int k = 0; k < nsub; ++k) store_agent(&gs->release[k].v, want);
+        released = true;
+      }
+    }
+    if (!released) {
+      unsigned int spins = 0;
+      while (load_agent(&gs->release[sub].v) != want) {
+        ++spins;
+        if ((spins & 63u) == 0 && load_agent(&gs->error.v)) break;
+        if (spins > spin_limit) {
+          store_agent(&gs->error.v, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    s_ok = load_agent(&gs->error.v) == 0;
+  }
+  __syncthreads();
+  gen += 1;
+  return s_ok != 0;
+}
+
+// agent-scope (device-coherent) accesses used inside the persistent kernel
+template <typename T>
+__device__ __forceinline__ T ld_coherent(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_coherent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------- persistent sync steps
+// nsteps synchronous steps in one cooperative launch: two grid barriers per
+// step (after the gathers, after the scatters).  Each group owns at most one
+// slot per step (the host sizes the grid so that tau <= groups).  The first
+// R*G entries of the slot's segment stay in registers from the gather to the
+// scatter, and the next step's coordinate, segment bounds and cached entries
+// -- none of which depend on the residual -- are loaded before the second
+// barrier, so only the residual gather and x_j remain on the critical path.
+constexpr int kPersistentBlock = 1024;  // one block per CU: 256 barrier arrivals
+
+template <typename T, int G, int LOSS, int R>
+__global__ __launch_bounds__(kPersistentBlock) void k_sync_persistent(
+    StepArgs<T> a, uint64_t step0, uint64_t nsteps, GridSync *gs, unsigned int spin_limit) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t slot = ((int64_t)blockIdx.x * kPersistentBlock + threadIdx.x) / G;
+  const bool active = slot < a.tau;  // group-uniform
+  unsigned int gen = 0;
+  uint32_t j = 0;
+  int32_t p0 = 0, p1 = 0;
+  int32_t ci[R];
+  T cv[R];
+  auto prefetch = [&](uint64_t step) {
+    j = a.perm((uint32_t)slot, step_keys(a.seed, step));
+    p0 = a.A.ptr[j];
+    p1 = a.A.ptr[j + 1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t p = p0 + lane + r * G;
+      ci[r] = p < p1 ? a.A.idx[p] : 0;
+      cv[r] = p < p1 ? a.A.val[p] : (T)0;
+    }
+  };
+  if (active) prefetch(step0);
+  for (uint64_t s = 0; s < nsteps; ++s) {
+    double c = 0.0;
+    if (active) {
+      T acc = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc += cv[r] * ld_coherent(&a.res[ci[r]]);
+      for (int32_t p = p0 + lane + R * G; p < p1; p += G)
+        acc += a.A.val[p] * ld_coherent(&a.res[a.A.idx[p]]);
+      acc = group_sum<G>(acc);
+      const double xj = ld_coherent(&a.x[j]);
+      const double d = coord_delta<T, LOSS>(a, j, (double)acc, xj, &c);
+      if (lane == 0) st_coherent(&a.x[j], xj + d);
+    }
+    if (!grid_sync(gs, gen, spin_limit)) return;
+    if (active) {
+      if (c != 0.0) {
+        const T ct = (T)c;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (p0 + lane + r * G < p1) unsafeAtomicAdd(&a.res[ci[r]], ct * cv[r]);
+        for (int32_t p = p0 + lane + R * G; p < p1; p += G)
+          unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
+      }
+      if (s + 1 < nsteps) prefetch(step0 + s + 1);
+    }
+    if (!grid_sync(gs, gen, spin_limit)) return;
+  }
 }
 
 // ------------------------------------------------------------- serial path
@@ -87,31 +266,25 @@ __global__ __launch_bounds__(kBlock) void k_step_scatter(
 // MulticoreSVMDualLoss.h:119-141).  Residual and x traffic uses agent-scope
 // atomics and a fence per update so that every update sees all earlier ones.
 template <typename T, int LOSS>
-__global__ __launch_bounds__(64) void k_serial(
-    Seg<T> A, T *__restrict__ res, double *__restrict__ x,
-    const double *__restrict__ L, const double *__restrict__ y, StepScalars sc,
-    Perm perm, uint64_t seed, uint64_t step0, uint64_t nsteps) {
+__global__ __launch_bounds__(64) void k_serial(StepArgs<T> a, uint64_t step0, uint64_t nsteps) {
   const int lane = threadIdx.x;
   for (uint64_t s = 0; s < nsteps; ++s) {
-    const StepKeys key = step_keys(seed, step0 + s);
-    const uint32_t j = perm(0u, key);
-    const int32_t p0 = A.ptr[j], p1 = A.ptr[j + 1];
+    const uint32_t j = a.perm(0u, step_keys(a.seed, step0 + s));
+    const int32_t p0 = a.A.ptr[j], p1 = a.A.ptr[j + 1];
     T acc = 0;
     for (int32_t p = p0 + lane; p < p1; p += 64)
-      acc += A.val[p] * __hip_atomic_load(&res[A.idx[p]], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
+      acc += a.A.val[p] * __hip_atomic_load(&a.res[a.A.idx[p]], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
     acc = group_sum<64>(acc);
-    const double xj =
-        __hip_atomic_load(&x[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double xj = __hip_atomic_load(&a.x[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double c;
-    const double d =
-        prox_delta<LOSS>((double)acc, xj, L[j], LOSS == kSvm ? y[j] : 0.0, sc, &c);
+    const double d = coord_delta<T, LOSS>(a, j, (double)acc, xj, &c);
     if (lane == 0)
-      __hip_atomic_store(&x[j], xj + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&a.x[j], xj + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (c != 0.0) {
       const T ct = (T)c;
       for (int32_t p = p0 + lane; p < p1; p += 64)
-        __hip_atomic_fetch_add(&res[A.idx[p]], ct * A.val[p], __ATOMIC_RELAXED,
+        __hip_atomic_fetch_add(&a.res[a.A.idx[p]], ct * a.A.val[p], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
     __threadfence();
@@ -122,8 +295,7 @@ __global__ __launch_bounds__(64) void k_serial(
 // L_c = 1/(beta ||segment c||^2) in FP64, 0 for an empty segment
 // (loss_square.h:183-194, MulticoreSVMDualLoss.h:48-61).
 template <typename T, int G>
-__global__ __launch_bounds__(kBlock) void k_lipschitz(Seg<T> A, int64_t nseg,
-                                                      double beta,
+__global__ __launch_bounds__(kBlock) void k_lipschitz(Seg<T> A, int64_t nseg, double beta,
                                                       double *__restrict__ L) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t c = t / G;
@@ -139,79 +311,55 @@ __global__ __launch_bounds__(kBlock) void k_lipschitz(Seg<T> A, int64_t nseg,
 }
 
 // ----------------------------------------------------------------- refresh
-// FP64 residual from the coordinates, one group per residual entry over the
-// residual-major copy (no atomics, fixed summation order):
-//   Lasso/NNLS  r_k = sum_j a_kj x_j - b_k          (MulticoreSquareLoss.h:42-53)
-//   SVM         w_f = sum_i x_if y_i alpha_i/(lambda n), alpha clipped to [0,1]
-//               (MulticoreSVMDualLoss.h:31-46)
-// res64 receives the FP64 value, resT (if not null) its rounding to T.
+// r64 = -b (square losses) or 0 (SVM w), then the nonzero coordinates are
+// scattered in with FP64 atomics (column-scatter form of
+// MulticoreSquareLoss::recomputeResiduals, MulticoreSquareLoss.h:42-53, and of
+// MulticoreSVMDualLoss.h:31-46 with alpha clipped to [0,1]).  Only segments
+// of nonzero coordinates are read, so a sparse x costs O(n + nnz(active)).
+__global__ __launch_bounds__(kBlock) void k_res_init(int64_t nres, const double *__restrict__ b,
+                                                     double *__restrict__ r64) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k < nres) r64[k] = b ? -b[k] : 0.0;
+}
+
 template <typename T, int G, int LOSS>
-__global__ __launch_bounds__(kBlock) void k_refresh(
-    Seg<T> R, int64_t nres, const double *__restrict__ x,
-    const double *__restrict__ b, const double *__restrict__ y,
-    double inv_lambda_n, double *__restrict__ res64, T *__restrict__ resT) {
+__global__ __launch_bounds__(kBlock) void k_refresh_scatter(Seg<T> A, int64_t nseg,
+                                                            const double *__restrict__ x,
+                                                            const double *__restrict__ y,
+                                                            double inv_lambda_n,
+                                                            double *__restrict__ r64) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t k = t / G;
+  const int64_t c = t / G;
   const int lane = threadIdx.x & (G - 1);
-  if (k >= nres) return;
-  double s = 0;
-  for (int32_t p = R.ptr[k] + lane; p < R.ptr[k + 1]; p += G) {
-    const int32_t c = R.idx[p];
-    double coefx;
-    if constexpr (LOSS == kSvm) {
-      double a = x[c];
-      a = a > 1.0 ? 1.0 : (a < 0.0 ? 0.0 : a);
-      coefx = a * y[c] * inv_lambda_n;
-    } else {
-      coefx = x[c];
-    }
-    s += (double)R.val[p] * coefx;
+  if (c >= nseg) return;
+  double coefx = x[c];
+  if constexpr (LOSS == kSvm) {
+    coefx = coefx > 1.0 ? 1.0 : (coefx < 0.0 ? 0.0 : coefx);
+    coefx *= y[c] * inv_lambda_n;
   }
-  s = group_sum<G>(s);
-  if (lane == 0) {
-    const double r = (LOSS == kSvm) ? s : s - b[k];
-    res64[k] = r;
-    if (resT) resT[k] = (T)r;
-  }
+  if (coefx == 0.0) return;
+  for (int32_t p = A.ptr[c] + lane; p < A.ptr[c + 1]; p += G)
+    unsafeAtomicAdd(&r64[A.idx[p]], (double)A.val[p] * coefx);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(T *__restrict__ v, int64_t n, T value) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k < n) v[k] = value;
 }
 
 // -------------------------------------------------------------- evaluation
-// Per-coordinate FP64 statistics with the FP64 residual, reduced to one
-// partial row of kStats doubles per block (fixed order; the host adds the
-// rows in order).
+// Per-coordinate FP64 statistics with the FP64 residual r64, one partial row
+// of kStats doubles per block (fixed order), then k_reduce_rows folds the
+// rows in a fixed tree:
 //   Lasso: {max|g_j|, sum|x_j|, nnz(x)}
 //   NNLS:  {max|min(x_j,g_j)|, sum x_j g_j, min g_j, max|g_j|, sum|x_j|, nnz}
 //   SVM:   {sum hinge, #(y_i m_i > 0), sum alpha_i, nnz(alpha)}
-// with g = A^T r (MulticoreL1RegSquareLoss.h:30-39 needs only ||x||_1; the
-// gradient feeds the dual point) and m_i = <x_i, w> (MulticoreSVMDualLoss.h:72-88).
+// with g = A^T r (the Lasso dual point needs ||A^T r||_inf) and
+// m_i = <x_i, w> (MulticoreSVMDualLoss.h:72-88).  If gref is not null the
+// per-coordinate g (m_i for the SVM) is stored there: the anchor of the FP32
+// path.
 constexpr int kStats = 8;
-
-template <int G>
-__device__ __forceinline__ void block_reduce_stats(double (&v)[kStats],
-                                                   const uint32_t maxmask,
-                                                   const uint32_t minmask,
-                                                   double *__restrict__ out) {
-  __shared__ double sh[kBlock / G][kStats];
-  const int grp = threadIdx.x / G;
-  const int lane = threadIdx.x & (G - 1);
-  if (lane == 0)
-    for (int s = 0; s < kStats; ++s) sh[grp][s] = v[s];
-  __syncthreads();
-  if (threadIdx.x < kStats) {
-    const int s = threadIdx.x;
-    double a = sh[0][s];
-    for (int g = 1; g < kBlock / G; ++g) {
-      const double o = sh[g][s];
-      if (maxmask & (1u << s))
-        a = o > a ? o : a;
-      else if (minmask & (1u << s))
-        a = o < a ? o : a;
-      else
-        a += o;
-    }
-    out[(int64_t)blockIdx.x * kStats + s] = a;
-  }
-}
 
 template <int LOSS>
 struct EvalMasks;
@@ -228,23 +376,32 @@ struct EvalMasks<kSvm> {
   static constexpr uint32_t max = 0, min = 0;
 };
 
+__device__ __forceinline__ double fold(double a, double o, int s, uint32_t maxmask,
+                                       uint32_t minmask) {
+  if (maxmask & (1u << s)) return o > a ? o : a;
+  if (minmask & (1u << s)) return o < a ? o : a;
+  return a + o;
+}
+
 template <typename T, int G, int LOSS>
-__global__ __launch_bounds__(kBlock) void k_eval_coord(
-    Seg<T> A, int64_t nseg, const double *__restrict__ res64,
-    const double *__restrict__ x, const double *__restrict__ y,
-    double *__restrict__ partial) {
+__global__ __launch_bounds__(kBlock) void k_eval_coord(Seg<T> A, int64_t nseg,
+                                                       const double *__restrict__ r64,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ y,
+                                                       double *__restrict__ gref,
+                                                       double *__restrict__ partial) {
   double v[kStats];
   for (int s = 0; s < kStats; ++s) v[s] = 0;
   if (LOSS == kNnls) v[2] = INFINITY;
   const int lane = threadIdx.x & (G - 1);
   const int64_t ngroups = (int64_t)gridDim.x * (kBlock / G);
-  for (int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G; c < nseg;
-       c += ngroups) {
+  for (int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G; c < nseg; c += ngroups) {
     double g = 0;
     for (int32_t p = A.ptr[c] + lane; p < A.ptr[c + 1]; p += G)
-      g += (double)A.val[p] * res64[A.idx[p]];
+      g += (double)A.val[p] * r64[A.idx[p]];
     g = group_sum<G>(g);
     if (lane == 0) {
+      if (gref) gref[c] = g;
       const double xc = x[c];
       if constexpr (LOSS == kLasso) {
         v[0] = fmax(v[0], fabs(g));
@@ -258,52 +415,85 @@ __global__ __launch_bounds__(kBlock) void k_eval_coord(
         v[4] += fabs(xc);
         v[5] += xc != 0.0;
       } else {
-        const double a = xc > 1.0 ? 1.0 : (xc < 0.0 ? 0.0 : xc);
+        const double al = xc > 1.0 ? 1.0 : (xc < 0.0 ? 0.0 : xc);
         const double ym = y[c] * g;
         const double h = 1.0 - ym;
         v[0] += h > 0 ? h : 0.0;
         v[1] += ym > 0;
-        v[2] += a;
-        v[3] += a != 0.0;
+        v[2] += al;
+        v[3] += al != 0.0;
       }
     }
   }
-  block_reduce_stats<G>(v, EvalMasks<LOSS>::max, EvalMasks<LOSS>::min, partial);
+  __shared__ double sh[kBlock / G][kStats];
+  const int grp = threadIdx.x / G;
+  if (lane == 0)
+    for (int s = 0; s < kStats; ++s) sh[grp][s] = v[s];
+  __syncthreads();
+  if (threadIdx.x < kStats) {
+    const int s = threadIdx.x;
+    double acc = sh[0][s];
+    for (int g2 = 1; g2 < kBlock / G; ++g2)
+      acc = fold(acc, sh[g2][s], s, EvalMasks<LOSS>::max, EvalMasks<LOSS>::min);
+    partial[(int64_t)blockIdx.x * kStats + s] = acc;
+  }
 }
 
 // Per-residual-entry statistics: {sum r_k^2, sum b_k r_k}.
-__global__ __launch_bounds__(kBlock) void k_eval_res(
-    int64_t nres, const double *__restrict__ res64,
-    const double *__restrict__ b, double *__restrict__ partial) {
-  double v[kStats];
-  for (int s = 0; s < kStats; ++s) v[s] = 0;
+__global__ __launch_bounds__(kBlock) void k_eval_res(int64_t nres, const double *__restrict__ r64,
+                                                     const double *__restrict__ b,
+                                                     double *__restrict__ partial) {
+  double v0 = 0, v1 = 0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nres; k += stride) {
-    const double r = res64[k];
-    v[0] += r * r;
-    if (b) v[1] += b[k] * r;
+    const double r = r64[k];
+    v0 += r * r;
+    if (b) v1 += b[k] * r;
   }
-  // every lane holds data here: reduce the wave first, then the block
-  for (int s = 0; s < 2; ++s) v[s] = group_sum<64>(v[s]);
+  v0 = group_sum<64>(v0);
+  v1 = group_sum<64>(v1);
   __shared__ double sh[kBlock / 64][2];
   if ((threadIdx.x & 63) == 0) {
-    sh[threadIdx.x / 64][0] = v[0];
-    sh[threadIdx.x / 64][1] = v[1];
+    sh[threadIdx.x / 64][0] = v0;
+    sh[threadIdx.x / 64][1] = v1;
   }
   __syncthreads();
   if (threadIdx.x < kStats) {
-    double a = 0;
+    double acc = 0;
     if (threadIdx.x < 2)
-      for (int w = 0; w < kBlock / 64; ++w) a += sh[w][threadIdx.x];
-    partial[(int64_t)blockIdx.x * kStats + threadIdx.x] = a;
+      for (int w = 0; w < kBlock / 64; ++w) acc += sh[w][threadIdx.x];
+    partial[(int64_t)blockIdx.x * kStats + threadIdx.x] = acc;
   }
+}
+
+// Folds nrows partial rows into out[kStats] with one block: thread t folds
+// rows t, t+256, ... in order, then a fixed pairwise tree.
+__global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
+                                                        int64_t nrows, uint32_t maxmask,
+                                                        uint32_t minmask,
+                                                        double *__restrict__ out) {
+  __shared__ double sh[kBlock][kStats];
+  double v[kStats];
+  for (int s = 0; s < kStats; ++s) {
+    v[s] = (minmask & (1u << s)) ? INFINITY : ((maxmask & (1u << s)) ? -INFINITY : 0.0);
+  }
+  for (int64_t r = threadIdx.x; r < nrows; r += kBlock)
+    for (int s = 0; s < kStats; ++s) v[s] = fold(v[s], partial[r * kStats + s], s, maxmask, minmask);
+  for (int s = 0; s < kStats; ++s) sh[threadIdx.x][s] = v[s];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int s = 0; s < kStats; ++s)
+        sh[threadIdx.x][s] = fold(sh[threadIdx.x][s], sh[threadIdx.x + w][s], s, maxmask, minmask);
+    __syncthreads();
+  }
+  if (threadIdx.x < kStats) out[threadIdx.x] = sh[0][threadIdx.x];
 }
 
 // --------------------------------------------------------------- sampling
 // out[s*tau + i] = perm_{seed, step0+s}(i): the device side of the host replay.
-__global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t seed,
-                                                   uint64_t step0, uint64_t nsteps,
-                                                   int64_t tau,
+__global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t seed, uint64_t step0,
+                                                   uint64_t nsteps, int64_t tau,
                                                    int32_t *__restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= (int64_t)nsteps * tau) return;
@@ -312,10 +502,9 @@ __global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t seed,
   out[t] = (int32_t)perm(i, step_keys(seed, step0 + s));
 }
 
-// Rescale the residual (SVM w) by a factor: w scales with 1/lambda.
+// v *= f (the SVM's w, its anchor and margins scale with 1/lambda).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scale(T *__restrict__ v, int64_t n,
-                                                  double f) {
+__global__ __launch_bounds__(kBlock) void k_scale(T *__restrict__ v, int64_t n, double f) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t < n) v[t] = (T)((double)v[t] * f);
 }

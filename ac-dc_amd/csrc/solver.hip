@@ -2,11 +2,16 @@
 // device data layout, kernel dispatch and the FP64 evaluation.
 //
 // Layout in HBM (one solver):
-//   A  coordinate-major segments  ptr int32[n+1], idx int32[nnz], val T[nnz]
-//   R  residual-major segments    ptr int32[m+1], idx int32[nnz], val T[nnz]
-//   x double[n], L double[n], b double[m] (Lasso/NNLS) or y double[n] (SVM)
-//   res T[m] (running residual r or w), res64 double[m] (FP64 refresh; the
-//   same buffer as res on the FP64 path), coord int32[tau], coef double[tau]
+//   A   coordinate-major segments  ptr int32[n+1], idx int32[nnz], val T[nnz]
+//       (FP32 path: plus val64 double[nnz], the FP64 values read by the
+//       refresh, evaluation and Lipschitz passes)
+//   x   double[n] (alpha for the SVM), L double[n] (1/(beta ||a_j||^2))
+//   rhs double[m] (b) or double[n] (SVM labels)
+//   res T[m]      FP64 path: running residual r (SVM: w)
+//                 FP32 path: correction dr since the last refresh
+//   r64 double[m] FP64 path: same buffer as res; FP32 path: anchor Ax - b
+//   gref double[n] FP32 path: A^T r64 at the last refresh
+//   coord int32[tau], coef double[tau]  staged between the two step phases
 // T = double on the FP64 path, float on the FP32+refresh path.
 #include <hip/hip_runtime.h>
 
@@ -16,6 +21,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/acdc_hip.h"
@@ -39,55 +45,40 @@ void set_error(const char *msg) { g_err = msg; }
 
 namespace {
 
-#define HIP_TRY(expr)                                                        \
-  do {                                                                       \
-    hipError_t e_ = (expr);                                                  \
-    if (e_ != hipSuccess)                                                    \
-      return fail(ACDC_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+#define HIP_TRY(expr)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(ACDC_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));   \
   } while (0)
 
 int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// bounded wait of the grid barrier: ~2^22 polls of ~50 ns
+constexpr unsigned int kSpinLimit = 1u << 22;
+// blocks of the evaluation grid (grid-stride beyond)
+constexpr int64_t kEvalBlocks = 8192;
+// steps per persistent launch
+constexpr uint64_t kStepsPerLaunch = 4096;
+// segment entries per lane kept in registers by the persistent kernel
+constexpr int kRegChunks = 2;
 }  // namespace
 
 // ----------------------------------------------------------------- problem
 struct acdc_problem {
-  int kind;          // 0: column-major input (Lasso/NNLS), 1: sample rows (SVM)
-  int64_t ncoord;    // coordinates: columns n, or samples
-  int64_t nres;      // residual entries: rows m, or features
+  int kind;        // 0: column-major input (Lasso/NNLS), 1: sample rows (SVM)
+  int64_t ncoord;  // coordinates: columns n, or samples
+  int64_t nres;    // residual entries: rows m, or features
   int64_t nnz;
-  std::vector<int32_t> cptr, cidx;  // coordinate-major
+  std::vector<int32_t> cptr, cidx;  // coordinate-major segments
   std::vector<double> cval;
-  std::vector<int32_t> rptr, ridx;  // residual-major
-  std::vector<double> rval;
   std::vector<double> rhs;  // b (m) or labels y (samples)
   int32_t omega;
 };
 
 namespace {
-// stable counting-sort transpose of a segment matrix (getCSR_from_CSC,
-// helpers/matrix_conversions.h:65-104)
-void transpose(int64_t nseg, int64_t nout, const std::vector<int32_t> &ptr,
-               const std::vector<int32_t> &idx, const std::vector<double> &val,
-               std::vector<int32_t> &optr, std::vector<int32_t> &oidx,
-               std::vector<double> &oval) {
-  const int64_t nnz = (int64_t)idx.size();
-  optr.assign(nout + 1, 0);
-  for (int64_t p = 0; p < nnz; ++p) optr[idx[p] + 1]++;
-  for (int64_t k = 0; k < nout; ++k) optr[k + 1] += optr[k];
-  std::vector<int32_t> fill(optr.begin(), optr.end() - 1);
-  oidx.resize(nnz);
-  oval.resize(nnz);
-  for (int64_t s = 0; s < nseg; ++s)
-    for (int32_t p = ptr[s]; p < ptr[s + 1]; ++p) {
-      const int32_t o = fill[idx[p]]++;
-      oidx[o] = (int32_t)s;
-      oval[o] = val[p];
-    }
-}
-
-acdc_status check_segments(int64_t nseg, int64_t nidx, int64_t nnz,
-                           const int32_t *ptr, const int32_t *idx,
-                           const char *what) {
+acdc_status check_segments(int64_t nseg, int64_t nidx, int64_t nnz, const int32_t *ptr,
+                           const int32_t *idx, const char *what) {
   if (nseg < 1 || nidx < 1 || nnz < 0)
     return fail(ACDC_ERR_INVALID, std::string(what) + ": sizes must be positive");
   if (nnz > INT32_MAX || nseg >= INT32_MAX || nidx >= INT32_MAX)
@@ -103,10 +94,39 @@ acdc_status check_segments(int64_t nseg, int64_t nidx, int64_t nnz,
   return ACDC_OK;
 }
 
-int32_t max_segment(const std::vector<int32_t> &ptr) {
-  int32_t om = 1;  // the reference starts omega at 1 (MultiCoreSolver.cpp:24)
-  for (size_t k = 0; k + 1 < ptr.size(); ++k) om = std::max(om, ptr[k + 1] - ptr[k]);
+// omega = max number of entries per index value; the reference starts its
+// running maximum at 1 (MultiCoreSolver.cpp:24, :32-38)
+int32_t max_count(int64_t nidx, const std::vector<int32_t> &idx) {
+  std::vector<int32_t> cnt(nidx, 0);
+  int32_t om = 1;
+  for (int32_t k : idx) om = std::max(om, ++cnt[k]);
   return om;
+}
+
+acdc_status make_problem(int kind, int64_t nseg, int64_t nidx, int64_t nnz,
+                         const int32_t *ptr, const int32_t *idx, const double *vals,
+                         const double *rhs, int64_t nrhs, acdc_problem **out,
+                         const char *what) {
+  acdc_status st = check_segments(nseg, nidx, nnz, ptr, idx, what);
+  if (st != ACDC_OK) return st;
+  acdc_problem *p = new (std::nothrow) acdc_problem();
+  if (!p) return fail(ACDC_ERR_NOMEM, std::string(what) + ": out of host memory");
+  try {
+    p->kind = kind;
+    p->ncoord = nseg;
+    p->nres = nidx;
+    p->nnz = nnz;
+    p->cptr.assign(ptr, ptr + nseg + 1);
+    p->cidx.assign(idx, idx + nnz);
+    p->cval.assign(vals, vals + nnz);
+    p->rhs.assign(rhs, rhs + nrhs);
+    p->omega = max_count(nidx, p->cidx);
+  } catch (const std::bad_alloc &) {
+    delete p;
+    return fail(ACDC_ERR_NOMEM, std::string(what) + ": out of host memory");
+  }
+  *out = p;
+  return ACDC_OK;
 }
 }  // namespace
 
@@ -128,66 +148,26 @@ extern "C" acdc_status acdc_device_count(int32_t *count) {
 }
 
 extern "C" acdc_status acdc_problem_create_csc(int64_t m, int64_t n, int64_t nnz,
-                                               const int32_t *col_ptr,
-                                               const int32_t *row_idx,
-                                               const double *vals,
-                                               const double *b,
+                                               const int32_t *col_ptr, const int32_t *row_idx,
+                                               const double *vals, const double *b,
                                                acdc_problem **out) {
   if (!out || !col_ptr || (nnz > 0 && (!row_idx || !vals)) || !b)
     return fail(ACDC_ERR_INVALID, "acdc_problem_create_csc: null argument");
-  acdc_status st = check_segments(n, m, nnz, col_ptr, row_idx, "acdc_problem_create_csc");
-  if (st != ACDC_OK) return st;
-  acdc_problem *p = new (std::nothrow) acdc_problem();
-  if (!p) return fail(ACDC_ERR_NOMEM, "acdc_problem_create_csc: out of host memory");
-  try {
-    p->kind = 0;
-    p->ncoord = n;
-    p->nres = m;
-    p->nnz = nnz;
-    p->cptr.assign(col_ptr, col_ptr + n + 1);
-    p->cidx.assign(row_idx, row_idx + nnz);
-    p->cval.assign(vals, vals + nnz);
-    p->rhs.assign(b, b + m);
-    transpose(n, m, p->cptr, p->cidx, p->cval, p->rptr, p->ridx, p->rval);
-    p->omega = max_segment(p->rptr);
-  } catch (const std::bad_alloc &) {
-    delete p;
-    return fail(ACDC_ERR_NOMEM, "acdc_problem_create_csc: out of host memory");
-  }
-  *out = p;
-  return ACDC_OK;
+  return make_problem(0, n, m, nnz, col_ptr, row_idx, vals, b, m, out,
+                      "acdc_problem_create_csc");
 }
 
 extern "C" acdc_status acdc_problem_create_csr(int64_t n_samples, int64_t n_features,
                                                int64_t nnz, const int32_t *row_ptr,
-                                               const int32_t *col_idx,
-                                               const double *vals,
-                                               const double *labels,
-                                               acdc_problem **out) {
+                                               const int32_t *col_idx, const double *vals,
+                                               const double *labels, acdc_problem **out) {
   if (!out || !row_ptr || (nnz > 0 && (!col_idx || !vals)) || !labels)
     return fail(ACDC_ERR_INVALID, "acdc_problem_create_csr: null argument");
-  acdc_status st = check_segments(n_samples, n_features, nnz, row_ptr, col_idx,
-                                  "acdc_problem_create_csr");
-  if (st != ACDC_OK) return st;
-  acdc_problem *p = new (std::nothrow) acdc_problem();
-  if (!p) return fail(ACDC_ERR_NOMEM, "acdc_problem_create_csr: out of host memory");
-  try {
-    p->kind = 1;
-    p->ncoord = n_samples;
-    p->nres = n_features;
-    p->nnz = nnz;
-    p->cptr.assign(row_ptr, row_ptr + n_samples + 1);
-    p->cidx.assign(col_idx, col_idx + nnz);
-    p->cval.assign(vals, vals + nnz);
-    p->rhs.assign(labels, labels + n_samples);
-    transpose(n_samples, n_features, p->cptr, p->cidx, p->cval, p->rptr, p->ridx, p->rval);
-    p->omega = max_segment(p->rptr);
-  } catch (const std::bad_alloc &) {
-    delete p;
-    return fail(ACDC_ERR_NOMEM, "acdc_problem_create_csr: out of host memory");
-  }
-  *out = p;
-  return ACDC_OK;
+  for (int64_t i = 0; i < n_samples; ++i)
+    if (labels[i] != 1.0 && labels[i] != -1.0)
+      return fail(ACDC_ERR_INVALID, "acdc_problem_create_csr: labels must be +1 or -1");
+  return make_problem(1, n_samples, n_features, nnz, row_ptr, col_idx, vals, labels,
+                      n_samples, out, "acdc_problem_create_csr");
 }
 
 extern "C" acdc_status acdc_problem_destroy(acdc_problem *p) {
@@ -202,29 +182,28 @@ extern "C" acdc_status acdc_problem_omega(const acdc_problem *p, int32_t *omega)
 }
 
 // ------------------------------------------------------------------ solver
-struct DevSeg {
-  int32_t *ptr = nullptr, *idx = nullptr;
-  void *val = nullptr;
-};
-
 struct acdc_solver {
   int loss = 0;
   acdc_settings set{};
   int64_t ncoord = 0, nres = 0, nnz = 0;
   double beta = 1.0;
-  int lanes = 64;
+  int lanes = 64;       // lanes per coordinate in the step kernels
+  int eval_lanes = 64;  // lanes per coordinate in refresh/evaluation
   bool fp32 = false;
+  bool persistent = false;
+  int64_t coresident_blocks = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DevSeg A, R;
+  int32_t *ptr = nullptr, *idx = nullptr;
+  void *val = nullptr;
+  double *val64 = nullptr;  // FP64 values (same buffer as val on the FP64 path)
   double *x = nullptr, *L = nullptr, *rhs = nullptr;
   void *res = nullptr;
-  double *res64 = nullptr;
+  double *r64 = nullptr, *gref = nullptr;
   int32_t *coord = nullptr;
   double *coef = nullptr;
-  double *partial = nullptr;
-  int32_t *samples = nullptr;
-  int64_t nblk_eval = 0;
+  double *partial = nullptr, *stats = nullptr;
+  GridSync *gsync = nullptr;
   uint64_t step = 0;
   uint64_t updates = 0;
   uint64_t updates_since_refresh = 0;
@@ -233,16 +212,10 @@ struct acdc_solver {
 };
 
 namespace {
-template <typename T>
-Seg<T> seg(const DevSeg &d) {
-  return Seg<T>{d.ptr, d.idx, static_cast<const T *>(d.val)};
-}
-
 acdc_status dalloc(acdc_solver *s, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
   hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess)
-    return fail(ACDC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return fail(ACDC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   s->allocs.push_back(*p);
   return ACDC_OK;
 }
@@ -255,28 +228,6 @@ acdc_status dalloc_t(acdc_solver *s, P **p, size_t count) {
   return st;
 }
 
-acdc_status upload_seg(acdc_solver *s, DevSeg &d, const std::vector<int32_t> &ptr,
-                       const std::vector<int32_t> &idx,
-                       const std::vector<double> &val) {
-  acdc_status st;
-  if ((st = dalloc_t(s, &d.ptr, ptr.size())) != ACDC_OK) return st;
-  if ((st = dalloc_t(s, &d.idx, idx.size())) != ACDC_OK) return st;
-  HIP_TRY(hipMemcpy(d.ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
-  if (!idx.empty())
-    HIP_TRY(hipMemcpy(d.idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
-  if (s->fp32) {
-    std::vector<float> f(val.begin(), val.end());
-    if ((st = dalloc(s, &d.val, f.size() * 4)) != ACDC_OK) return st;
-    if (!f.empty())
-      HIP_TRY(hipMemcpy(d.val, f.data(), f.size() * 4, hipMemcpyHostToDevice));
-  } else {
-    if ((st = dalloc(s, &d.val, val.size() * 8)) != ACDC_OK) return st;
-    if (!val.empty())
-      HIP_TRY(hipMemcpy(d.val, val.data(), val.size() * 8, hipMemcpyHostToDevice));
-  }
-  return ACDC_OK;
-}
-
 StepScalars scalars(const acdc_solver *s) {
   StepScalars sc;
   sc.lambda = s->set.lambda;
@@ -284,6 +235,27 @@ StepScalars scalars(const acdc_solver *s) {
   sc.inv_lambda_n = 1.0 / sc.lambda_n;
   return sc;
 }
+
+template <typename T>
+StepArgs<T> step_args(const acdc_solver *s) {
+  StepArgs<T> a;
+  a.A = Seg<T>{s->ptr, s->idx, static_cast<const T *>(s->val)};
+  a.res = static_cast<T *>(s->res);
+  a.gref = s->fp32 ? s->gref : nullptr;
+  a.x = s->x;
+  a.L = s->L;
+  a.y = s->loss == kSvm ? s->rhs : nullptr;
+  a.sc = scalars(s);
+  a.perm = Perm::make((uint32_t)s->ncoord);
+  a.seed = s->set.seed;
+  a.tau = s->set.tau;
+  a.coord = s->coord;
+  a.coef = s->coef;
+  return a;
+}
+
+// the FP64 view of A used by the refresh, evaluation and setup passes
+Seg<double> seg64(const acdc_solver *s) { return Seg<double>{s->ptr, s->idx, s->val64}; }
 
 int auto_lanes(int64_t nnz, int64_t nseg) {
   const double avg = nseg > 0 ? (double)nnz / (double)nseg : 0.0;
@@ -293,9 +265,9 @@ int auto_lanes(int64_t nnz, int64_t nseg) {
   return 64;
 }
 
-// ---- dispatch helpers: value type x lanes x loss
+// ---- dispatch: value type x lanes x loss
 template <template <typename, int, int> class F, typename... Args>
-void dispatch3(const acdc_solver *s, Args &&...args) {
+void dispatch3(const acdc_solver *s, int lanes, Args &&...args) {
   auto go = [&](auto tag_t, auto tag_g) {
     using T = typename decltype(tag_t)::type;
     constexpr int G = decltype(tag_g)::value;
@@ -306,7 +278,7 @@ void dispatch3(const acdc_solver *s, Args &&...args) {
     }
   };
   auto by_g = [&](auto tag_t) {
-    switch (s->lanes) {
+    switch (lanes) {
       case 8: go(tag_t, std::integral_constant<int, 8>{}); break;
       case 16: go(tag_t, std::integral_constant<int, 16>{}); break;
       case 32: go(tag_t, std::integral_constant<int, 32>{}); break;
@@ -323,26 +295,43 @@ void dispatch3(const acdc_solver *s, Args &&...args) {
 template <typename T, int G, int LOSS>
 struct LaunchStep {
   static void run(const acdc_solver *s, uint64_t step, int part) {
-    const int64_t tau = s->set.tau;
-    const unsigned blocks = (unsigned)div_up(tau * G, kBlock);
-    if (part & 1)
-      k_step_gather<T, G, LOSS><<<blocks, kBlock, 0, s->stream>>>(
-          seg<T>(s->A), static_cast<const T *>(s->res), s->x, s->L, s->rhs,
-          scalars(s), Perm::make((uint32_t)s->ncoord), s->set.seed, step, tau,
-          s->coord, s->coef);
-    if (part & 2)
-      k_step_scatter<T, G><<<blocks, kBlock, 0, s->stream>>>(
-          seg<T>(s->A), static_cast<T *>(s->res), s->coord, s->coef, tau);
+    const unsigned blocks = (unsigned)div_up(s->set.tau * G, kBlock);
+    const StepArgs<T> a = step_args<T>(s);
+    if (part & 1) k_step_gather<T, G, LOSS><<<blocks, kBlock, 0, s->stream>>>(a, step);
+    if (part & 2) k_step_scatter<T, G><<<blocks, kBlock, 0, s->stream>>>(a);
   }
 };
 
 template <typename T, int G, int LOSS>
 struct LaunchSerial {
   static void run(const acdc_solver *s, uint64_t step0, uint64_t nsteps) {
-    const Perm perm = Perm::make((uint32_t)s->ncoord);
-    k_serial<T, LOSS><<<1, 64, 0, s->stream>>>(
-        seg<T>(s->A), static_cast<T *>(s->res), s->x, s->L, s->rhs, scalars(s),
-        perm, s->set.seed, step0, nsteps);
+    k_serial<T, LOSS><<<1, 64, 0, s->stream>>>(step_args<T>(s), step0, nsteps);
+  }
+};
+
+template <typename T, int G, int LOSS>
+struct PersistentKernel {
+  static const void *fn() {
+    return reinterpret_cast<const void *>(&k_sync_persistent<T, G, LOSS, kRegChunks>);
+  }
+  static void run(const acdc_solver *s, uint64_t step0, uint64_t nsteps, hipError_t *err) {
+    StepArgs<T> a = step_args<T>(s);
+    GridSync *gs = s->gsync;
+    unsigned int spin = kSpinLimit;
+    void *args[] = {&a, &step0, &nsteps, &gs, &spin};
+    const unsigned blocks = (unsigned)div_up(s->set.tau * G, kPersistentBlock);
+    *err = hipLaunchCooperativeKernel(fn(), dim3(blocks), dim3(kPersistentBlock), args, 0,
+                                      s->stream);
+  }
+};
+
+template <typename T, int G, int LOSS>
+struct Occupancy {
+  static void run(const acdc_solver *s, int *blocks_per_cu) {
+    (void)s;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, PersistentKernel<T, G, LOSS>::fn(),
+                                                     kPersistentBlock, 0) != hipSuccess)
+      *blocks_per_cu = 0;
   }
 };
 
@@ -350,41 +339,42 @@ template <typename T, int G, int LOSS>
 struct LaunchLipschitz {
   static void run(const acdc_solver *s) {
     const unsigned blocks = (unsigned)div_up(s->ncoord * G, kBlock);
-    k_lipschitz<T, G><<<blocks, kBlock, 0, s->stream>>>(seg<T>(s->A), s->ncoord,
-                                                        s->beta, s->L);
+    k_lipschitz<double, G><<<blocks, kBlock, 0, s->stream>>>(seg64(s), s->ncoord, s->beta, s->L);
   }
 };
 
 template <typename T, int G, int LOSS>
-struct LaunchRefresh {
-  static void run(const acdc_solver *s, bool write_back) {
-    const unsigned blocks = (unsigned)div_up(s->nres * G, kBlock);
-    T *resT = write_back && static_cast<void *>(s->res64) != s->res
-                  ? static_cast<T *>(s->res)
-                  : nullptr;
-    k_refresh<T, G, LOSS><<<blocks, kBlock, 0, s->stream>>>(
-        seg<T>(s->R), s->nres, s->x, s->rhs, s->rhs, scalars(s).inv_lambda_n,
-        s->res64, resT);
+struct LaunchRefreshScatter {
+  static void run(const acdc_solver *s, double *target) {
+    const unsigned blocks = (unsigned)div_up(s->ncoord * G, kBlock);
+    k_refresh_scatter<double, G, LOSS><<<blocks, kBlock, 0, s->stream>>>(
+        seg64(s), s->ncoord, s->x, s->rhs, scalars(s).inv_lambda_n, target);
   }
 };
 
 template <typename T, int G, int LOSS>
 struct LaunchEvalCoord {
-  static void run(const acdc_solver *s) {
-    k_eval_coord<T, G, LOSS><<<(unsigned)s->nblk_eval, kBlock, 0, s->stream>>>(
-        seg<T>(s->A), s->ncoord, s->res64, s->x, s->rhs, s->partial);
+  static void run(const acdc_solver *s, double *gref) {
+    const int64_t blocks = std::min(kEvalBlocks, div_up(s->ncoord * G, kBlock));
+    k_eval_coord<double, G, LOSS><<<(unsigned)blocks, kBlock, 0, s->stream>>>(
+        seg64(s), s->ncoord, s->r64, s->x, s->rhs, gref, s->partial);
+    k_reduce_rows<<<1, kBlock, 0, s->stream>>>(s->partial, blocks, EvalMasks<LOSS>::max,
+                                               EvalMasks<LOSS>::min, s->stats);
   }
 };
 
-// residual-major lanes: chosen from the residual segments' mean length
-int res_lanes(const acdc_solver *s) { return auto_lanes(s->nnz, s->nres); }
-
-acdc_status launch_refresh(acdc_solver *s, bool write_back) {
-  const int keep = s->lanes;
-  s->lanes = res_lanes(s);
-  dispatch3<LaunchRefresh>(s, write_back);
-  s->lanes = keep;
+// r64 = A x - b (SVM: w) from the nonzero coordinates; FP32 path: then
+// gref = A^T r64 with the evaluation statistics in stats[0..7], dr = 0.
+acdc_status refresh(acdc_solver *s) {
+  const unsigned rb = (unsigned)div_up(s->nres, kBlock);
+  k_res_init<<<rb, kBlock, 0, s->stream>>>(s->nres, s->loss == kSvm ? nullptr : s->rhs, s->r64);
+  dispatch3<LaunchRefreshScatter>(s, s->eval_lanes, s->r64);
+  if (s->fp32) {
+    dispatch3<LaunchEvalCoord>(s, s->eval_lanes, s->gref);
+    k_fill<float><<<rb, kBlock, 0, s->stream>>>(static_cast<float *>(s->res), s->nres, 0.0f);
+  }
   HIP_TRY(hipGetLastError());
+  s->updates_since_refresh = 0;
   return ACDC_OK;
 }
 
@@ -410,22 +400,48 @@ void destroy_solver(acdc_solver *s) {
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
+
+// steps [step0, step0 + nsteps) in persistent launches of kStepsPerLaunch.
+// A refused cooperative launch runs nothing; in ACDC_LAUNCH_AUTO the solver
+// then switches to per-step launches (same arithmetic) for good.
+acdc_status run_persistent(acdc_solver *s, uint64_t step0, uint64_t nsteps) {
+  for (uint64_t done = 0; done < nsteps;) {
+    const uint64_t chunk = std::min(kStepsPerLaunch, nsteps - done);
+    HIP_TRY(hipMemsetAsync(s->gsync, 0, sizeof(GridSync), s->stream));
+    hipError_t err = hipSuccess;
+    dispatch3<PersistentKernel>(s, s->lanes, step0 + done, chunk, &err);
+    if (err != hipSuccess) {
+      if (s->set.launch_mode != ACDC_LAUNCH_AUTO)
+        return fail(ACDC_ERR_HIP, std::string("cooperative launch: ") + hipGetErrorString(err));
+      (void)hipGetLastError();
+      s->persistent = false;
+      for (uint64_t k = done; k < nsteps; ++k) dispatch3<LaunchStep>(s, s->lanes, step0 + k, 3);
+      return ACDC_OK;
+    }
+    done += chunk;
+  }
+  return ACDC_OK;
+}
 }  // namespace
 
 extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
-                                          const acdc_settings *set,
-                                          acdc_solver **out) {
+                                          const acdc_settings *set, acdc_solver **out) {
   if (!p || !set || !out) return fail(ACDC_ERR_INVALID, "acdc_solver_create: null argument");
   if (loss != kLasso && loss != kNnls && loss != kSvm)
-    return fail(ACDC_ERR_INVALID, "acdc_solver_create: loss must be 0 (L1-square), 2 (SVM dual hinge) or 10 (NNLS)");
+    return fail(ACDC_ERR_INVALID,
+                "acdc_solver_create: loss must be 0 (L1-square), 2 (SVM dual hinge) or 10 (NNLS)");
   if ((loss == kSvm) != (p->kind == 1))
-    return fail(ACDC_ERR_STATE, "acdc_solver_create: SVM needs a sample-major (CSR) problem, losses 0/10 a column-major (CSC) one");
+    return fail(ACDC_ERR_STATE,
+                "acdc_solver_create: SVM needs a sample-major (CSR) problem, losses 0/10 a "
+                "column-major (CSC) one");
   if (set->tau < 1 || set->tau > p->ncoord)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: tau must lie in [1, n]");
   if (set->precision != ACDC_PREC_FP64 && set->precision != ACDC_PREC_FP32_REFRESH)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown precision");
   if (set->step_mode != ACDC_STEP_SYNC)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown step mode");
+  if (set->launch_mode < ACDC_LAUNCH_AUTO || set->launch_mode > ACDC_LAUNCH_PERSISTENT)
+    return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown launch mode");
   if (set->lanes_per_coord != 0 && set->lanes_per_coord != 8 && set->lanes_per_coord != 16 &&
       set->lanes_per_coord != 32 && set->lanes_per_coord != 64)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: lanes_per_coord must be 0, 8, 16, 32 or 64");
@@ -433,6 +449,21 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: lambda must be >= 0 (> 0 for the SVM)");
   if (set->refresh_period < 0)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: refresh_period must be >= 0");
+  double beta = 1.0;
+  switch (set->beta_mode) {
+    case ACDC_BETA_OMEGA:
+      beta = p->ncoord > 1 ? 1.0 + (double)(p->omega - 1) * (double)(set->tau - 1) /
+                                       ((double)p->ncoord - 1.0)
+                           : 1.0;
+      break;
+    case ACDC_BETA_FORCED:
+      if (!(set->forced_beta > 0))
+        return fail(ACDC_ERR_INVALID, "acdc_solver_create: forced_beta must be > 0");
+      beta = set->forced_beta;
+      break;
+    case ACDC_BETA_NONE: beta = 1.0; break;
+    default: return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown beta_mode");
+  }
   int32_t ndev = 0;
   acdc_device_count(&ndev);
   if (ndev < 1) return fail(ACDC_ERR_NO_DEVICE, "acdc_solver_create: no gfx950 device visible");
@@ -444,56 +475,82 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
   s->ncoord = p->ncoord;
   s->nres = p->nres;
   s->nnz = p->nnz;
+  s->beta = beta;
   s->fp32 = set->precision == ACDC_PREC_FP32_REFRESH;
   s->lanes = set->lanes_per_coord ? set->lanes_per_coord : auto_lanes(p->nnz, p->ncoord);
-  switch (set->beta_mode) {
-    case ACDC_BETA_OMEGA:
-      s->beta = p->ncoord > 1 ? 1.0 + (double)(p->omega - 1) * (double)(set->tau - 1) /
-                                          ((double)p->ncoord - 1.0)
-                              : 1.0;
-      break;
-    case ACDC_BETA_FORCED:
-      if (!(set->forced_beta > 0)) {
-        delete s;
-        return fail(ACDC_ERR_INVALID, "acdc_solver_create: forced_beta must be > 0");
-      }
-      s->beta = set->forced_beta;
-      break;
-    case ACDC_BETA_NONE: s->beta = 1.0; break;
-    default: delete s; return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown beta_mode");
-  }
+  s->eval_lanes = auto_lanes(p->nnz, p->ncoord);
 
-  acdc_status st = ACDC_OK;
   auto bail = [&](acdc_status e) {
     std::string msg = g_err;
     destroy_solver(s);
     g_err = msg;
     return e;
   };
+  acdc_status st = ACDC_OK;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return bail(fail(ACDC_ERR_HIP, "acdc_solver_create: stream/event creation failed"));
-  if ((st = upload_seg(s, s->A, p->cptr, p->cidx, p->cval)) != ACDC_OK) return bail(st);
-  if ((st = upload_seg(s, s->R, p->rptr, p->ridx, p->rval)) != ACDC_OK) return bail(st);
   const size_t tsz = s->fp32 ? 4 : 8;
+  if ((st = dalloc_t(s, &s->ptr, p->cptr.size())) != ACDC_OK) return bail(st);
+  if ((st = dalloc_t(s, &s->idx, p->cidx.size())) != ACDC_OK) return bail(st);
+  if ((st = dalloc(s, &s->val, p->cval.size() * tsz)) != ACDC_OK) return bail(st);
+  if (s->fp32) {
+    if ((st = dalloc_t(s, &s->val64, p->cval.size())) != ACDC_OK) return bail(st);
+  } else {
+    s->val64 = static_cast<double *>(s->val);
+  }
   if ((st = dalloc_t(s, &s->x, s->ncoord)) != ACDC_OK) return bail(st);
   if ((st = dalloc_t(s, &s->L, s->ncoord)) != ACDC_OK) return bail(st);
   if ((st = dalloc_t(s, &s->rhs, p->rhs.size())) != ACDC_OK) return bail(st);
   if ((st = dalloc(s, &s->res, s->nres * tsz)) != ACDC_OK) return bail(st);
   if (s->fp32) {
-    if ((st = dalloc_t(s, &s->res64, s->nres)) != ACDC_OK) return bail(st);
+    if ((st = dalloc_t(s, &s->r64, s->nres)) != ACDC_OK) return bail(st);
+    if ((st = dalloc_t(s, &s->gref, s->ncoord)) != ACDC_OK) return bail(st);
   } else {
-    s->res64 = static_cast<double *>(s->res);
+    s->r64 = static_cast<double *>(s->res);
   }
   if ((st = dalloc_t(s, &s->coord, set->tau)) != ACDC_OK) return bail(st);
   if ((st = dalloc_t(s, &s->coef, set->tau)) != ACDC_OK) return bail(st);
-  s->nblk_eval = 1024;
-  if ((st = dalloc_t(s, &s->partial, (size_t)s->nblk_eval * kStats)) != ACDC_OK) return bail(st);
-  if (hipMemcpy(s->rhs, p->rhs.data(), p->rhs.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(s->x, 0, s->ncoord * 8) != hipSuccess)
-    return bail(fail(ACDC_ERR_HIP, "acdc_solver_create: upload failed"));
-  dispatch3<LaunchLipschitz>(s);
-  if ((st = launch_refresh(s, true)) != ACDC_OK) return bail(st);
+  if ((st = dalloc_t(s, &s->partial, (size_t)kEvalBlocks * kStats)) != ACDC_OK) return bail(st);
+  if ((st = dalloc_t(s, &s->stats, 2 * kStats)) != ACDC_OK) return bail(st);
+  if ((st = dalloc_t(s, &s->gsync, 1)) != ACDC_OK) return bail(st);
+
+  hipError_t e = hipMemcpy(s->ptr, p->cptr.data(), p->cptr.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && p->nnz > 0)
+    e = hipMemcpy(s->idx, p->cidx.data(), p->cidx.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && p->nnz > 0) {
+    if (s->fp32) {
+      std::vector<float> f(p->cval.begin(), p->cval.end());
+      e = hipMemcpy(s->val, f.data(), f.size() * 4, hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(s->val64, p->cval.data(), p->cval.size() * 8, hipMemcpyHostToDevice);
+    } else {
+      e = hipMemcpy(s->val, p->cval.data(), p->cval.size() * 8, hipMemcpyHostToDevice);
+    }
+  }
+  if (e == hipSuccess) e = hipMemcpy(s->rhs, p->rhs.data(), p->rhs.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(s->x, 0, s->ncoord * 8);
+  if (e != hipSuccess)
+    return bail(fail(ACDC_ERR_HIP, std::string("acdc_solver_create: upload: ") + hipGetErrorString(e)));
+
+  // persistent grid: every slot's group must be resident at once
+  if (set->tau >= 2 && set->launch_mode != ACDC_LAUNCH_PER_STEP) {
+    int per_cu = 0, dev = 0;
+    hipDeviceProp_t prop;
+    (void)hipGetDevice(&dev);
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+      dispatch3<Occupancy>(s, s->lanes, &per_cu);
+      s->coresident_blocks = (int64_t)per_cu * prop.multiProcessorCount;
+    }
+    const int64_t need = div_up(set->tau * s->lanes, kPersistentBlock);
+    s->persistent = need <= s->coresident_blocks;
+    if (!s->persistent && set->launch_mode == ACDC_LAUNCH_PERSISTENT)
+      return bail(fail(ACDC_ERR_INVALID,
+                       "acdc_solver_create: tau * lanes_per_coord exceeds the co-resident grid "
+                       "of the persistent kernel"));
+  }
+  dispatch3<LaunchLipschitz>(s, s->eval_lanes);
+  if ((st = refresh(s)) != ACDC_OK) return bail(st);
   if (hipStreamSynchronize(s->stream) != hipSuccess)
     return bail(fail(ACDC_ERR_HIP, "acdc_solver_create: setup kernels failed"));
   *out = s;
@@ -522,29 +579,34 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
       chunk = std::min(chunk, (left + tau - 1) / tau);
     }
     if (tau == 1) {
-      dispatch3<LaunchSerial>(s, s->step, chunk);
+      dispatch3<LaunchSerial>(s, s->lanes, s->step, chunk);
+    } else if (s->persistent) {
+      if ((st = run_persistent(s, s->step, chunk)) != ACDC_OK) return st;
     } else {
-      for (uint64_t k = 0; k < chunk; ++k) dispatch3<LaunchStep>(s, s->step + k, 3);
+      for (uint64_t k = 0; k < chunk; ++k) dispatch3<LaunchStep>(s, s->lanes, s->step + k, 3);
     }
     s->step += chunk;
     s->updates += chunk * tau;
     s->updates_since_refresh += chunk * tau;
     done += chunk;
-    if (period && s->updates_since_refresh >= period) {
-      if ((st = launch_refresh(s, true)) != ACDC_OK) return st;
-      s->updates_since_refresh = 0;
-    }
+    if (period && s->updates_since_refresh >= period)
+      if ((st = refresh(s)) != ACDC_OK) return st;
   }
   HIP_TRY(hipGetLastError());
-  return timed_end(s, &s->kernel_ms);
+  if ((st = timed_end(s, &s->kernel_ms)) != ACDC_OK) return st;
+  if (s->persistent) {
+    GridSync g{};
+    HIP_TRY(hipMemcpy(&g, s->gsync, sizeof g, hipMemcpyDeviceToHost));
+    if (g.error.v) return fail(ACDC_ERR_HIP, "acdc_run_updates: grid barrier wait exceeded its bound");
+  }
+  return ACDC_OK;
 }
 
 extern "C" acdc_status acdc_recompute_residuals(acdc_solver *s) {
   if (!s) return fail(ACDC_ERR_INVALID, "acdc_recompute_residuals: null solver");
   acdc_status st;
   if ((st = timed_begin(s)) != ACDC_OK) return st;
-  if ((st = launch_refresh(s, true)) != ACDC_OK) return st;
-  s->updates_since_refresh = 0;
+  if ((st = refresh(s)) != ACDC_OK) return st;
   return timed_end(s, &s->eval_ms);
 }
 
@@ -553,66 +615,47 @@ extern "C" acdc_status acdc_report_get(acdc_solver *s, acdc_report *out) {
   acdc_status st;
   if ((st = timed_begin(s)) != ACDC_OK) return st;
   if (s->fp32) {
-    if ((st = launch_refresh(s, true)) != ACDC_OK) return st;
-    s->updates_since_refresh = 0;
+    if ((st = refresh(s)) != ACDC_OK) return st;  // stats come with the anchor
+  } else {
+    dispatch3<LaunchEvalCoord>(s, s->eval_lanes, nullptr);
   }
-  dispatch3<LaunchEvalCoord>(s);
+  const int64_t rblocks = std::min(kEvalBlocks, div_up(s->nres, kBlock));
+  k_eval_res<<<(unsigned)rblocks, kBlock, 0, s->stream>>>(
+      s->nres, s->r64, s->loss == kSvm ? nullptr : s->rhs, s->partial);
+  k_reduce_rows<<<1, kBlock, 0, s->stream>>>(s->partial, rblocks, 0u, 0u, s->stats + kStats);
   HIP_TRY(hipGetLastError());
-  std::vector<double> pc((size_t)s->nblk_eval * kStats), pr((size_t)s->nblk_eval * kStats);
-  HIP_TRY(hipMemcpyAsync(pc.data(), s->partial, pc.size() * 8, hipMemcpyDeviceToHost, s->stream));
-  k_eval_res<<<(unsigned)s->nblk_eval, kBlock, 0, s->stream>>>(
-      s->nres, s->res64, s->loss == kSvm ? nullptr : s->rhs, s->partial);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(pr.data(), s->partial, pr.size() * 8, hipMemcpyDeviceToHost, s->stream));
+  double h[2 * kStats];
+  HIP_TRY(hipMemcpyAsync(h, s->stats, sizeof h, hipMemcpyDeviceToHost, s->stream));
   if ((st = timed_end(s, &s->eval_ms)) != ACDC_OK) return st;
-
-  // host: fixed-order compensated sums of the block rows
-  auto reduce = [&](const std::vector<double> &v, int k, int how) {
-    double sum = 0, c = 0, ext = how == 1 ? -INFINITY : INFINITY;
-    for (int64_t b = 0; b < s->nblk_eval; ++b) {
-      const double a = v[(size_t)b * kStats + k];
-      if (how == 1) {
-        ext = std::max(ext, a);
-      } else if (how == 2) {
-        ext = std::min(ext, a);
-      } else {
-        const double t = sum + a;
-        c += std::fabs(sum) >= std::fabs(a) ? (sum - t) + a : (a - t) + sum;
-        sum = t;
-      }
-    }
-    return how == 0 ? sum + c : ext;
-  };
+  const double *pc = h, *pr = h + kStats;
   acdc_report r{};
-  const double R2 = reduce(pr, 0, 0), bTr = reduce(pr, 1, 0);
+  const double R2 = pr[0], bTr = pr[1];
   const double lam = s->set.lambda;
   if (s->loss == kLasso) {
-    const double ginf = reduce(pc, 0, 1), x1 = reduce(pc, 1, 0);
+    const double ginf = pc[0], x1 = pc[1];
     r.primal = 0.5 * R2 + lam * x1;
     const double sc = ginf > lam ? lam / ginf : 1.0;
     r.dual = -sc * bTr - 0.5 * sc * sc * R2;
     r.grad_inf = ginf;
     r.x_l1 = x1;
     r.res_sq = R2;
-    r.nnz_x = reduce(pc, 2, 0);
+    r.nnz_x = pc[2];
   } else if (s->loss == kNnls) {
-    const double xg = reduce(pc, 1, 0);
     r.primal = 0.5 * R2;
-    r.dual = r.primal - xg;
-    r.kkt = reduce(pc, 0, 1);
-    r.grad_inf = reduce(pc, 3, 1);
-    r.x_l1 = reduce(pc, 4, 0);
+    r.dual = r.primal - pc[1];
+    r.kkt = pc[0];
+    r.grad_inf = pc[3];
+    r.x_l1 = pc[4];
     r.res_sq = R2;
-    r.nnz_x = reduce(pc, 5, 0);
+    r.nnz_x = pc[5];
   } else {
     const double n = (double)s->ncoord;
-    const double hinge = reduce(pc, 0, 0), good = reduce(pc, 1, 0), sa = reduce(pc, 2, 0);
-    r.primal = lam * 0.5 * R2 + hinge / n;
-    r.dual = -lam * 0.5 * R2 + sa / n;
-    r.accuracy = good / n;
-    r.x_l1 = sa;
+    r.primal = lam * 0.5 * R2 + pc[0] / n;
+    r.dual = -lam * 0.5 * R2 + pc[2] / n;
+    r.accuracy = pc[1] / n;
+    r.x_l1 = pc[2];
     r.res_sq = R2;
-    r.nnz_x = reduce(pc, 3, 0);
+    r.nnz_x = pc[3];
   }
   r.gap = r.primal - r.dual;
   r.rel_gap = r.primal != 0 ? r.gap / std::fabs(r.primal) : 0.0;
@@ -637,9 +680,8 @@ extern "C" acdc_status acdc_set_x(acdc_solver *s, const double *x, int64_t n) {
   if (n != s->ncoord) return fail(ACDC_ERR_INVALID, "acdc_set_x: length must equal n");
   HIP_TRY(hipStreamSynchronize(s->stream));
   HIP_TRY(hipMemcpy(s->x, x, n * 8, hipMemcpyHostToDevice));
-  acdc_status st = launch_refresh(s, true);
+  acdc_status st = refresh(s);
   if (st != ACDC_OK) return st;
-  s->updates_since_refresh = 0;
   HIP_TRY(hipStreamSynchronize(s->stream));
   return ACDC_OK;
 }
@@ -648,10 +690,11 @@ extern "C" acdc_status acdc_get_residuals(acdc_solver *s, double *r, int64_t len
   if (!s || !r) return fail(ACDC_ERR_INVALID, "acdc_get_residuals: null argument");
   if (len != s->nres) return fail(ACDC_ERR_INVALID, "acdc_get_residuals: wrong length");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  if (s->fp32) {
+  if (s->fp32) {  // r = r64 + dr
     std::vector<float> f((size_t)len);
     HIP_TRY(hipMemcpy(f.data(), s->res, len * 4, hipMemcpyDeviceToHost));
-    for (int64_t k = 0; k < len; ++k) r[k] = f[k];
+    HIP_TRY(hipMemcpy(r, s->r64, len * 8, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < len; ++k) r[k] += f[k];
   } else {
     HIP_TRY(hipMemcpy(r, s->res, len * 8, hipMemcpyDeviceToHost));
   }
@@ -671,13 +714,17 @@ extern "C" acdc_status acdc_set_lambda(acdc_solver *s, double lambda) {
   if (!(lambda >= 0) || (s->loss == kSvm && !(lambda > 0)))
     return fail(ACDC_ERR_INVALID, "acdc_set_lambda: lambda must be >= 0 (> 0 for the SVM)");
   if (s->loss == kSvm && lambda != s->set.lambda) {
+    // w = (1/(lambda n)) sum alpha_i y_i x_i: w, its anchor and the anchored
+    // margins all scale by lambda_old / lambda_new
     const double f = s->set.lambda / lambda;
-    const unsigned blocks = (unsigned)div_up(s->nres, kBlock);
+    const unsigned rb = (unsigned)div_up(s->nres, kBlock);
     if (s->fp32) {
-      k_scale<float><<<blocks, kBlock, 0, s->stream>>>(static_cast<float *>(s->res), s->nres, f);
-      k_scale<double><<<blocks, kBlock, 0, s->stream>>>(s->res64, s->nres, f);
+      k_scale<float><<<rb, kBlock, 0, s->stream>>>(static_cast<float *>(s->res), s->nres, f);
+      k_scale<double><<<rb, kBlock, 0, s->stream>>>(s->r64, s->nres, f);
+      k_scale<double><<<(unsigned)div_up(s->ncoord, kBlock), kBlock, 0, s->stream>>>(
+          s->gref, s->ncoord, f);
     } else {
-      k_scale<double><<<blocks, kBlock, 0, s->stream>>>(s->res64, s->nres, f);
+      k_scale<double><<<rb, kBlock, 0, s->stream>>>(s->r64, s->nres, f);
     }
     HIP_TRY(hipGetLastError());
   }
@@ -691,8 +738,8 @@ extern "C" acdc_status acdc_get_beta(const acdc_solver *s, double *beta) {
   return ACDC_OK;
 }
 
-extern "C" acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0,
-                                               uint64_t nsteps, int32_t *out) {
+extern "C" acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0, uint64_t nsteps,
+                                               int32_t *out) {
   if (!s || !out) return fail(ACDC_ERR_INVALID, "acdc_sample_coordinates: null argument");
   const int64_t total = (int64_t)nsteps * s->set.tau;
   if (total <= 0) return ACDC_OK;
@@ -706,7 +753,8 @@ extern "C" acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0,
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)total * 4, hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(d);
-  if (e != hipSuccess) return fail(ACDC_ERR_HIP, std::string("acdc_sample_coordinates: ") + hipGetErrorString(e));
+  if (e != hipSuccess)
+    return fail(ACDC_ERR_HIP, std::string("acdc_sample_coordinates: ") + hipGetErrorString(e));
   return ACDC_OK;
 }
 
@@ -716,8 +764,8 @@ extern "C" acdc_status acdc_synchronize(acdc_solver *s) {
   return ACDC_OK;
 }
 
-extern "C" acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps,
-                                                 double *gather_ms, double *scatter_ms) {
+extern "C" acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps, double *gather_ms,
+                                                 double *scatter_ms) {
   if (!s || !gather_ms || !scatter_ms)
     return fail(ACDC_ERR_INVALID, "acdc_profile_step_kernels: null argument");
   if (s->set.tau < 2)
@@ -732,9 +780,9 @@ extern "C" acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps
   if (st == ACDC_OK) {
     for (uint64_t k = 0; k < nsteps; ++k) {
       (void)hipEventRecord(ev[3 * k], s->stream);
-      dispatch3<LaunchStep>(s, s->step, 1);
+      dispatch3<LaunchStep>(s, s->lanes, s->step, 1);
       (void)hipEventRecord(ev[3 * k + 1], s->stream);
-      dispatch3<LaunchStep>(s, s->step, 2);
+      dispatch3<LaunchStep>(s, s->lanes, s->step, 2);
       (void)hipEventRecord(ev[3 * k + 2], s->stream);
       s->step += 1;
       s->updates += (uint64_t)s->set.tau;

@@ -1,0 +1,99 @@
+// acdc_hip_solver -- console front end of the gfx950 PCDM path.
+//
+// Counterpart of cpp/src/frontends/solvers/MultiCoreSolver.cpp:18-115 with
+// the getopt letters of cpp/src/helpers/option_console_parser.h:102-114:
+//   -A <prefix|file>  text CSC prefix (losses 0/10) or LIBSVM file (loss 2)
+//   -b <file>         right-hand side b (losses 0/10)
+//   -l <lambda>       regularisation
+//   -t <tau>          coordinates per synchronous step (totalThreads)
+//   -f <loss>         0 L1-square (Lasso), 2 SVM dual hinge, 10 NNLS
+//   -I <n>            updates per bulk (innerIterations)
+//   -c <n>            number of bulks (the reference stores -c into a bool)
+//   -i <0|1>          show initial/intermediate/last objective values
+//   -r <file>         CSV log file
+// and the HIP knobs (no reference counterpart):
+//   -s <seed>  -p <0 fp64 | 1 fp32+refresh>  -R <refresh epochs>  -L <lanes>
+//   -P <0 auto | 1 per-step launches | 2 persistent grid>
+// In the reference, -I falls through into -M (option_console_parser.h:89-93);
+// here each flag sets only its own field.  omega and beta are computed as in
+// MultiCoreSolver.cpp:24-38, :55-62, :108-110.
+#include <getopt.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+
+#include "../host/acdc/HipEngineExecutor.h"
+#include "../host/acdc/Solver.h"
+#include "../host/acdc/file_reader.h"
+
+static void usage() {
+  std::cerr << "usage: acdc_hip_solver -A <data> [-b <b file>] -l <lambda> [-t tau] [-f loss]\n"
+               "                       [-I updates per bulk] [-c bulks] [-i 0|1] [-r log]\n"
+               "                       [-s seed] [-p precision] [-R refresh epochs] [-L lanes]\n"
+               "                       [-P launch mode]\n";
+}
+
+int main(int argc, char *argv[]) {
+  OptimizationSettings settings;
+  settings.verbose = true;
+  std::string matrixA, vectorB, logPath;
+  double lambda = 0;
+  int c;
+  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:s:p:R:L:P:h")) != -1) {
+    switch (c) {
+      case 'A': matrixA = optarg; break;
+      case 'b': vectorB = optarg; break;
+      case 'l': lambda = std::atof(optarg); break;
+      case 't': settings.totalThreads = std::atoi(optarg); break;
+      case 'f': settings.lossFunction = std::atoi(optarg); break;
+      case 'I': settings.innerIterations = std::strtoul(optarg, nullptr, 10); break;
+      case 'c': settings.iters_bulkIterations_count = std::strtoul(optarg, nullptr, 10); break;
+      case 'i': {
+        const bool v = std::atoi(optarg) != 0;
+        settings.showIntermediateObjectiveValue = v;
+        settings.showInitialObjectiveValue = v;
+        settings.showLastObjectiveValue = v;
+        break;
+      }
+      case 'r': logPath = optarg; break;
+      case 's': settings.hipSeed = std::strtoull(optarg, nullptr, 10); break;
+      case 'p': settings.hipPrecision = std::atoi(optarg); break;
+      case 'R': settings.hipRefreshPeriod = std::atoi(optarg); break;
+      case 'L': settings.hipLanesPerCoord = std::atoi(optarg); break;
+      case 'P': settings.hipLaunchMode = std::atoi(optarg); break;
+      default: usage(); return c == 'h' ? 0 : 2;
+    }
+  }
+  if (matrixA.empty()) {
+    usage();
+    return 2;
+  }
+  std::ofstream logFile;
+  if (!logPath.empty()) {
+    logFile.open(logPath.c_str());
+    settings.logToFile = true;
+    settings.logFile = &logFile;
+  }
+  try {
+    ProblemData<int, double> instance;
+    if (settings.lossFunction == ACDC_LOSS_SVM_DUAL_HINGE) {
+      InputOuputHelper::loadLibSVMData(matrixA, instance);
+      instance.lambda = lambda;
+    } else {
+      if (vectorB.empty()) throw std::runtime_error("-b <b file> is required for losses 0 and 10");
+      InputOuputHelper::loadCSCData(matrixA, vectorB, lambda, instance);
+    }
+    std::cout << "Data loaded with size: " << instance.m << " x " << instance.n << std::endl;
+    HipEngineExecutor<int, double> executor(instance, &settings);
+    std::cout << "ESO beta: " << instance.sigma << std::endl;
+    Solver<int, double> solver(executor);
+    solver.runSolver();
+  } catch (const std::exception &e) {
+    std::cerr << "acdc_hip_solver: " << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,167 @@
+// HipEngineExecutor.h -- the gfx950 executor behind the reference's engine
+// interface.
+//
+// AbstractEngineExecutor<L,D> restates cpp/src/solver/AbstractEngineExecutor.h:13-39;
+// HipEngineExecutor<L,D> takes the place of MulticoreEngineExecutor<L,D>
+// (cpp/src/solver/MulticoreEngineExecutor.h:25-97): same constructor
+// arguments, the same side effects on the caller's ProblemData (x zeroed and
+// kept current, Li filled), the same four operations, but every operation is
+// a call into the C ABI of include/acdc_hip.h.  Errors, which the reference
+// never reports, surface as std::runtime_error carrying acdc_last_error().
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/acdc_hip.h"
+#include "losses.h"
+#include "structures.h"
+
+template <typename L, typename D>
+class AbstractEngineExecutor {
+ public:
+  OptimizationSettings *settings = nullptr;
+  virtual ~AbstractEngineExecutor() {}
+  virtual std::string getResultLogHeaders() { return "Function is not implemented"; }
+  virtual std::string getResultLogRow(OptimizationStatistics &) { return "Function is not implemented"; }
+  virtual void executeBulkOfIterations() {}
+  virtual void recomputeResiduals() {}
+  virtual D getObjectiveValue() { return 0; }
+};
+
+namespace acdc_host {
+
+inline void check(acdc_status st, const char *what) {
+  if (st != ACDC_OK) throw std::runtime_error(std::string(what) + ": " + acdc_last_error());
+}
+
+template <typename L>
+std::vector<int32_t> to_i32(const std::vector<L> &v, const char *what) {
+  std::vector<int32_t> out(v.size());
+  for (size_t i = 0; i < v.size(); ++i) {
+    const long long e = (long long)v[i];
+    if (e < std::numeric_limits<int32_t>::min() || e > std::numeric_limits<int32_t>::max())
+      throw std::runtime_error(std::string(what) + ": index exceeds int32");
+    out[i] = (int32_t)e;
+  }
+  return out;
+}
+
+template <typename D>
+std::vector<double> to_f64(const std::vector<D> &v) {
+  return std::vector<double>(v.begin(), v.end());
+}
+
+}  // namespace acdc_host
+
+template <typename L, typename D>
+class HipEngineExecutor : public AbstractEngineExecutor<L, D> {
+ public:
+  ProblemData<L, D> &instance;
+
+  HipEngineExecutor(ProblemData<L, D> &_instance, OptimizationSettings *_settings)
+      : instance(_instance) {
+    using namespace acdc_host;
+    this->settings = _settings;
+    const int loss = _settings->lossFunction;
+    acdc_status st;
+    if (loss == ACDC_LOSS_SVM_DUAL_HINGE) {
+      const std::vector<int32_t> rp = to_i32(instance.A_csr_row_ptr, "A_csr_row_ptr");
+      const std::vector<int32_t> ci = to_i32(instance.A_csr_col_idx, "A_csr_col_idx");
+      const std::vector<double> v = to_f64(instance.A_csr_values), y = to_f64(instance.b);
+      st = acdc_problem_create_csr((int64_t)instance.n, (int64_t)instance.m, (int64_t)ci.size(),
+                                   rp.data(), ci.data(), v.data(), y.data(), &problem_);
+    } else {
+      const std::vector<int32_t> cp = to_i32(instance.A_csc_col_ptr, "A_csc_col_ptr");
+      const std::vector<int32_t> ri = to_i32(instance.A_csc_row_idx, "A_csc_row_idx");
+      const std::vector<double> v = to_f64(instance.A_csc_values), b = to_f64(instance.b);
+      st = acdc_problem_create_csc((int64_t)instance.m, (int64_t)instance.n, (int64_t)ri.size(),
+                                   cp.data(), ri.data(), v.data(), b.data(), &problem_);
+    }
+    check(st, "acdc_problem_create");
+    acdc_settings s{};
+    s.lambda = (double)instance.lambda;
+    s.tau = std::max(1, _settings->totalThreads);
+    s.seed = _settings->hipSeed;
+    // an explicit beta wins: forcedSigma, else the caller's instance.sigma
+    // (MultiCoreSolver.cpp:108-110 fills it), else omega and tau
+    if (_settings->forcedSigma > 0) {
+      s.beta_mode = ACDC_BETA_FORCED;
+      s.forced_beta = _settings->forcedSigma;
+    } else if (instance.sigma > 0) {
+      s.beta_mode = ACDC_BETA_FORCED;
+      s.forced_beta = (double)instance.sigma;
+    } else {
+      s.beta_mode = ACDC_BETA_OMEGA;
+    }
+    s.precision = _settings->hipPrecision;
+    s.step_mode = ACDC_STEP_SYNC;
+    s.lanes_per_coord = _settings->hipLanesPerCoord;
+    s.refresh_period = _settings->hipRefreshPeriod;
+    s.launch_mode = _settings->hipLaunchMode;
+    st = acdc_solver_create(problem_, loss, &s, &solver_);
+    if (st != ACDC_OK) {
+      acdc_problem_destroy(problem_);
+      check(st, "acdc_solver_create");
+    }
+    double beta = 1;
+    acdc_get_beta(solver_, &beta);
+    instance.sigma = (D)beta;
+    instance.x.assign((size_t)instance.n, (D)0);
+    std::vector<double> li((size_t)instance.n);
+    check(acdc_get_lipschitz(solver_, li.data(), (int64_t)li.size()), "acdc_get_lipschitz");
+    instance.Li.assign(li.begin(), li.end());
+  }
+
+  ~HipEngineExecutor() override {
+    acdc_solver_destroy(solver_);
+    acdc_problem_destroy(problem_);
+  }
+  HipEngineExecutor(const HipEngineExecutor &) = delete;
+  HipEngineExecutor &operator=(const HipEngineExecutor &) = delete;
+
+  void executeBulkOfIterations() override {
+    acdc_host::check(acdc_run_updates(solver_, (uint64_t)this->settings->innerIterations),
+                     "acdc_run_updates");
+    pull_x();
+  }
+
+  void recomputeResiduals() override {
+    acdc_host::check(acdc_recompute_residuals(solver_), "acdc_recompute_residuals");
+  }
+
+  // Lasso/NNLS: the objective (MulticoreL1RegSquareLoss.h:30-39); SVM: the
+  // duality gap P - D (MulticoreSVMDualLoss.h:114-116)
+  D getObjectiveValue() override {
+    acdc_host::check(acdc_report_get(solver_, &last_), "acdc_report_get");
+    instance.primalObjective = (D)last_.primal;
+    instance.dualObjective = (D)last_.dual;
+    instance.oneZeroAccuracy = (D)last_.accuracy;
+    return (D)(this->settings->lossFunction == ACDC_LOSS_SVM_DUAL_HINGE ? last_.gap : last_.primal);
+  }
+
+  std::string getResultLogHeaders() override {
+    return acdc_losses::log_headers(this->settings->lossFunction);
+  }
+  std::string getResultLogRow(OptimizationStatistics &statistics) override {
+    return acdc_losses::log_row(this->settings->lossFunction, last_, statistics);
+  }
+
+  const acdc_report &lastReport() const { return last_; }
+  acdc_solver *handle() { return solver_; }
+
+ private:
+  void pull_x() {
+    std::vector<double> x((size_t)instance.n);
+    acdc_host::check(acdc_get_x(solver_, x.data(), (int64_t)x.size()), "acdc_get_x");
+    instance.x.assign(x.begin(), x.end());
+  }
+
+  acdc_problem *problem_ = nullptr;
+  acdc_solver *solver_ = nullptr;
+  acdc_report last_{};
+};

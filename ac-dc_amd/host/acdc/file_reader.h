@@ -1,0 +1,96 @@
+// file_reader.h -- the on-disk formats of the path.
+//
+// Text CSC (InputOuputHelper::loadCSCData, cpp/src/utils/file_reader.h:108-137):
+//   <prefix>_values.txt, <prefix>_colPtr.txt, <prefix>_rowIdx.txt and a b file,
+//   whitespace-separated; n = |colPtr| - 1, m = |b|.  Unlike the reference's
+//   loadVectorData (:33-48), a final value without a trailing newline is kept,
+//   and a missing file is an error rather than an empty vector.
+// LIBSVM rows (loadDistributedSparseSVMRowData,
+//   cpp/src/utils/distributed_instances_loader.h:68-193): "label idx:val ..."
+//   per sample, 1-based feature indices shifted to 0-based, label 0 read as -1;
+//   m = largest feature index, n = samples.  Unlike the reference, labels >= 100
+//   and malformed tokens are errors instead of silently skipped rows.
+#pragma once
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "structures.h"
+
+namespace InputOuputHelper {
+
+template <typename G>
+void loadVectorData(const std::string &fileLocation, std::vector<G> &vector) {
+  std::ifstream f(fileLocation.c_str());
+  if (!f.is_open()) throw std::runtime_error("file not found: " + fileLocation);
+  vector.clear();
+  G val;
+  while (f >> val) vector.push_back(val);
+  if (!f.eof()) throw std::runtime_error("malformed number in " + fileLocation);
+}
+
+template <typename L, typename D>
+void loadCSCData(const std::string &matrixPrefix, const std::string &vectorBFileName,
+                 double lambda, ProblemData<L, D> &instance) {
+  loadVectorData(vectorBFileName, instance.b);
+  loadVectorData(matrixPrefix + "_values.txt", instance.A_csc_values);
+  loadVectorData(matrixPrefix + "_colPtr.txt", instance.A_csc_col_ptr);
+  loadVectorData(matrixPrefix + "_rowIdx.txt", instance.A_csc_row_idx);
+  if (instance.A_csc_col_ptr.empty()) throw std::runtime_error("empty column pointer file");
+  instance.n = (L)(instance.A_csc_col_ptr.size() - 1);
+  instance.m = (L)instance.b.size();
+  instance.lambda = (D)lambda;
+}
+
+template <typename L, typename D>
+void saveCSCData(const std::string &matrixPrefix, const std::string &vectorBFileName,
+                 const ProblemData<L, D> &instance) {
+  auto dump = [](const std::string &path, const auto &v) {
+    std::ofstream f(path.c_str());
+    if (!f.is_open()) throw std::runtime_error("cannot write " + path);
+    f.precision(17);
+    for (const auto &e : v) f << e << "\n";
+  };
+  dump(vectorBFileName, instance.b);
+  dump(matrixPrefix + "_values.txt", instance.A_csc_values);
+  dump(matrixPrefix + "_colPtr.txt", instance.A_csc_col_ptr);
+  dump(matrixPrefix + "_rowIdx.txt", instance.A_csc_row_idx);
+}
+
+template <typename L, typename D>
+void loadLibSVMData(const std::string &file, ProblemData<L, D> &part) {
+  std::ifstream f(file.c_str());
+  if (!f.is_open()) throw std::runtime_error("file not found: " + file);
+  part.A_csr_row_ptr.assign(1, 0);
+  part.A_csr_col_idx.clear();
+  part.A_csr_values.clear();
+  part.b.clear();
+  long long maxFeature = 0;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream ls(line);
+    std::string tok;
+    if (!(ls >> tok)) continue;  // blank line
+    int label = 0;
+    if (std::sscanf(tok.c_str(), "%d", &label) != 1 || label >= 100)
+      throw std::runtime_error("bad label '" + tok + "' in " + file);
+    part.b.push_back(label == 0 ? (D)-1 : (D)(label > 0 ? 1 : -1));
+    while (ls >> tok) {
+      long long idx = 0;
+      double v = 0;
+      if (std::sscanf(tok.c_str(), "%lld:%lf", &idx, &v) != 2 || idx < 1)
+        throw std::runtime_error("bad feature '" + tok + "' in " + file);
+      part.A_csr_col_idx.push_back((L)(idx - 1));
+      part.A_csr_values.push_back((D)v);
+      if (idx > maxFeature) maxFeature = idx;
+    }
+    part.A_csr_row_ptr.push_back((L)part.A_csr_col_idx.size());
+  }
+  part.n = (L)part.b.size();
+  part.m = (L)maxFeature;
+}
+
+}  // namespace InputOuputHelper

@@ -8,10 +8,11 @@ of this benchmark is one epoch: n = 1,000,000 coordinate updates (245
 synchronous tau-steps) over data resident in HBM.
 
 Prints one JSON line: coordinate updates/s of the FP32 + FP64-refresh path
-(value), the same for the plain FP64 path, the time each needs to reach an
-FP64 relative duality gap <= 1e-6 (checked every 5 epochs), the roofline of
-the dominant kernel (phase-A gather), and the CPU baseline (the oracle's
-OpenMP asynchronous restatement of the reference solver) on the host cores.
+(value; persistent cooperative kernel), the same for the plain FP64 path
+(one gather + one scatter launch per step), the time each needs from x = 0
+to an FP64 relative duality gap <= 1e-6 (gap checked every 5 epochs), the
+roofline of the dominant kernel, and the CPU baseline (the oracle's OpenMP
+asynchronous restatement of the reference solver) on the host cores.
 """
 import argparse
 import json
@@ -30,8 +31,8 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_to_gap(acdc, solver, n, target, check_every, max_epochs):
-    """Epochs from x = 0 with an FP64 gap check every check_every epochs."""
+def time_to_gap(solver, n, target, check_every, max_epochs):
+    """Wall time from x = 0 to rel gap <= target, FP64 gap every check_every epochs."""
     import numpy as np
     solver.x = np.zeros(n)
     solver.synchronize()
@@ -44,6 +45,19 @@ def time_to_gap(acdc, solver, n, target, check_every, max_epochs):
     return time.perf_counter() - t0, epochs, rep
 
 
+def throughput(solver, n, epochs):
+    """Coordinate updates/s over `epochs` epochs (wall clock around them)."""
+    solver.synchronize()
+    u0 = solver.report()["updates"]
+    t0 = time.perf_counter()
+    for _ in range(epochs):
+        solver.run_updates(n)
+    solver.synchronize()
+    wall = time.perf_counter() - t0
+    upd = solver.report()["updates"] - u0
+    return upd / wall, wall, upd
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,7 +68,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--gap-target", type=float, default=1e-6)
     ap.add_argument("--gap-every", type=int, default=5)
-    ap.add_argument("--max-epochs", type=int, default=300)
+    ap.add_argument("--max-epochs", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttg", action="store_true", help="skip the time-to-gap runs")
@@ -72,40 +86,39 @@ def main():
     t = time.perf_counter()
     d = acdc.config_lasso_c2(m=args.m, n=args.n)
     log(f"[bench] generated {d.m} x {d.n}, nnz {d.nnz} in {time.perf_counter() - t:.1f}s")
-    t = time.perf_counter()
     prob = d.problem()
-    log(f"[bench] problem ready in {time.perf_counter() - t:.1f}s, omega {prob.omega}")
     n, tau = d.n, args.tau
     mean_nnz = d.nnz / d.n
 
-    def make(precision):
+    def make(precision, launch, refresh=5):
         return acdc.Solver(prob, acdc.LOSS_L1_SQUARE, lam=d.lam, tau=tau, seed=2,
-                           precision=precision, refresh_period=5)
+                           precision=precision, refresh_period=refresh, launch_mode=launch)
 
-    # ---- FP32 + FP64 refresh: throughput over the timed epochs
-    s32 = make(acdc.PREC_FP32_REFRESH)
+    # ---- FP32 + FP64 refresh, persistent grid: throughput over the timed epochs
+    s32 = make(acdc.PREC_FP32_REFRESH, acdc.LAUNCH_AUTO)
     beta = s32.beta
     s32.run_updates(args.warmup * n)
     s32.x = np.zeros(n)
-    s32.synchronize()
-    u0 = s32.report()["updates"]
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s32.run_updates(n)
-    s32.synchronize()
-    wall = time.perf_counter() - t0
-    rep32 = s32.report()
-    upd = rep32["updates"] - u0
-    value = upd / wall
-    log(f"[bench] fp32: {upd} updates in {wall:.3f}s = {value:.4e} upd/s, "
-        f"rel gap after {args.steps} epochs {rep32['rel_gap']:.3e}")
+    value, wall, upd = throughput(s32, n, args.steps)
+    log(f"[bench] fp32 persistent: {upd} updates in {wall:.3f}s = {value:.4e} upd/s")
+    s32.close()
 
-    # ---- dominant kernel: phase-A gather, timed with HIP events on its stream
-    gather_ms, scatter_ms = s32.profile_step_kernels(64)
+    # ---- dominant kernel: one persistent launch per epoch (refresh off),
+    #      timed with HIP events on the solver's stream
+    sk = make(acdc.PREC_FP32_REFRESH, acdc.LAUNCH_AUTO, refresh=0)
+    sk.run_updates(n)
+    k0 = sk.report()["kernel_ms"]
+    reps = 8
+    for _ in range(reps):
+        sk.run_updates(n)
+    launch_ms = (sk.report()["kernel_ms"] - k0) / reps
+    steps_per_launch = -(-n // tau)
     bytes_per_update = 8.0 * mean_nnz + 32.0  # idx+val per nnz, col ptrs, L_j, x_j r/w
-    achieved = tau * bytes_per_update / (gather_ms * 1e-3) / 1e9
-    log(f"[bench] gather {gather_ms * 1e3:.2f} us, scatter {scatter_ms * 1e3:.2f} us per launch; "
-        f"gather {achieved:.1f} GB/s")
+    bytes_per_launch = steps_per_launch * tau * bytes_per_update
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    log(f"[bench] k_sync_persistent: {launch_ms:.3f} ms per launch ({steps_per_launch} steps), "
+        f"{achieved:.1f} GB/s")
+    sk.close()
 
     result = {
         "metric": METRIC, "value": value, "unit": "coord updates/s", "n_gpus": 1,
@@ -114,47 +127,42 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"lasso {d.m}x{d.n} zipf1.1[5,200] tau{tau}", "m": d.m, "n": d.n,
                    "nnz": d.nnz, "omega": prob.omega, "tau": tau, "beta": beta,
-                   "lambda": d.lam, "precision": "fp32 dots/scatter + fp64 x, refresh every 5 epochs",
+                   "lambda": d.lam,
+                   "precision": "fp32 dots/scatter on an fp64-anchored residual, fp64 x, "
+                                "fp64 refresh every 5 epochs",
                    "step": "one epoch = n coordinate updates"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step_gather", "kernel_us": gather_ms * 1e3,
-                     "bytes_per_launch": tau * bytes_per_update},
+                     "kernel": "k_sync_persistent<float,64,0,2>", "launch_ms": launch_ms,
+                     "steps_per_launch": steps_per_launch,
+                     "bytes_per_update": bytes_per_update, "bytes_per_launch": bytes_per_launch},
     }
-    s32.close()
 
-    # ---- plain FP64 path, same steps
-    s64 = make(acdc.PREC_FP64)
+    # ---- plain FP64 path: per-step launches
+    s64 = make(acdc.PREC_FP64, acdc.LAUNCH_PER_STEP)
     s64.run_updates(args.warmup * n)
     s64.x = np.zeros(n)
-    s64.synchronize()
-    u0 = s64.report()["updates"]
-    t0 = time.perf_counter()
-    for _ in range(max(1, args.steps // 5)):
-        s64.run_updates(n)
-    s64.synchronize()
-    wall64 = time.perf_counter() - t0
-    v64 = (s64.report()["updates"] - u0) / wall64
-    result["fp64_plain"] = {"value": v64, "unit": "coord updates/s"}
-    log(f"[bench] fp64: {v64:.4e} upd/s")
+    v64, wall64, _ = throughput(s64, n, max(1, args.steps // 5))
+    result["fp64_plain"] = {"value": v64, "unit": "coord updates/s", "launch": "per step"}
+    log(f"[bench] fp64 plain: {v64:.4e} upd/s")
 
+    x32 = None
     if not args.no_ttg:
-        for name, prec in (("fp32_refresh", acdc.PREC_FP32_REFRESH), ("fp64_plain", acdc.PREC_FP64)):
-            s = s64 if prec == acdc.PREC_FP64 else make(prec)
-            tt, ep, rep = time_to_gap(acdc, s, n, args.gap_target, args.gap_every, args.max_epochs)
-            key = "time_to_gap" if prec == acdc.PREC_FP32_REFRESH else None
-            entry = {"seconds": tt, "epochs": ep, "rel_gap": rep["rel_gap"],
-                     "reached": rep["rel_gap"] <= args.gap_target}
-            if key:
-                result["time_to_gap"] = entry
-            else:
-                result["fp64_plain"]["time_to_gap"] = entry
-            log(f"[bench] {name}: rel gap {rep['rel_gap']:.3e} after {ep} epochs in {tt:.3f}s")
-            if prec == acdc.PREC_FP32_REFRESH:
-                x32 = s.x
-                s.close()
+        s = make(acdc.PREC_FP32_REFRESH, acdc.LAUNCH_AUTO)
+        tt, ep, rep = time_to_gap(s, n, args.gap_target, args.gap_every, args.max_epochs)
+        result["time_to_gap"] = {"seconds": tt, "epochs": ep, "rel_gap": rep["rel_gap"],
+                                 "reached": rep["rel_gap"] <= args.gap_target,
+                                 "check_every_epochs": args.gap_every}
+        log(f"[bench] fp32: rel gap {rep['rel_gap']:.3e} after {ep} epochs in {tt:.3f}s")
+        x32 = s.x
+        s.close()
+        tt, ep, rep = time_to_gap(s64, n, args.gap_target, args.gap_every, args.max_epochs)
+        result["fp64_plain"]["time_to_gap"] = {"seconds": tt, "epochs": ep,
+                                               "rel_gap": rep["rel_gap"],
+                                               "reached": rep["rel_gap"] <= args.gap_target}
+        log(f"[bench] fp64 plain: rel gap {rep['rel_gap']:.3e} after {ep} epochs in {tt:.3f}s")
         if result["time_to_gap"]["reached"] and result["fp64_plain"]["time_to_gap"]["reached"]:
-            result["time_to_gap"]["speedup_vs_fp64"] = (
+            result["time_to_gap"]["speedup_vs_fp64_plain"] = (
                 result["fp64_plain"]["time_to_gap"]["seconds"] / result["time_to_gap"]["seconds"])
     s64.close()
 
@@ -164,9 +172,8 @@ def main():
         threads = max(1, min(16, os.cpu_count() or 1))
         o = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, d.lam,
                    beta=orc.eso_beta(prob.omega, threads, d.n))
-        if not args.no_ttg:
-            e = o.eval(x32)
-            result["time_to_gap"]["rel_gap_fp64_oracle"] = e["rel_gap"]
+        if x32 is not None:
+            result["time_to_gap"]["rel_gap_fp64_oracle"] = o.eval(x32)["rel_gap"]
         t0 = time.perf_counter()
         done = 0
         while time.perf_counter() - t0 < args.cpu_seconds and done < 30 * n:
@@ -175,8 +182,9 @@ def main():
         cw = time.perf_counter() - t0
         result["cpu_baseline"] = {"value": done / cw, "unit": "coord updates/s", "cores": threads,
                                   "kind": "port",
-                                  "sample": f"{done} asynchronous FP64 updates ({done / n:.1f} epochs) "
-                                            f"of the same Lasso from x = 0, {cw:.1f}s, CAS atomics"}
+                                  "sample": f"{done} asynchronous FP64 updates ({done / n:.1f} "
+                                            f"epochs) of the same Lasso from x = 0 in {cw:.1f}s, "
+                                            "CAS atomics"}
         log(f"[bench] cpu: {done / cw:.4e} upd/s on {threads} threads")
     print(json.dumps(result), flush=True)
 

@@ -75,7 +75,16 @@ typedef struct acdc_settings {
                               sampled coordinate */
   int32_t refresh_period;  /* FP32 path: FP64 residual refresh every this
                               many epochs inside acdc_run_updates (0 = off) */
+  int32_t launch_mode;     /* acdc_launch_mode: how tau > 1 steps execute */
 } acdc_settings;
+
+/* Execution of synchronous steps with tau > 1; the arithmetic is the same. */
+typedef enum acdc_launch_mode {
+  ACDC_LAUNCH_AUTO = 0,      /* persistent cooperative grid when all tau slots
+                                fit on the device at once, else per step */
+  ACDC_LAUNCH_PER_STEP = 1,  /* one gather + one scatter launch per step */
+  ACDC_LAUNCH_PERSISTENT = 2 /* persistent grid; fails if it does not fit */
+} acdc_launch_mode;
 
 /* One evaluation, all in FP64.  Counterparts: SquareLoss::getResultLogRow
  * (squareLoss/SquareLoss.h:28-38) and SVMDualLoss::getResultLogRow
