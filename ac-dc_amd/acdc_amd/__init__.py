@@ -91,6 +91,19 @@ class _DatasetInfo(ctypes.Structure):
                 ("lambda_max", ctypes.c_double), ("xstar_nnz", ctypes.c_int64)]
 
 
+class BatchSettings(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("precision", ctypes.c_int32),
+                ("lanes_per_coord", ctypes.c_int32), ("refresh_period", ctypes.c_int32),
+                ("max_epochs", ctypes.c_int32), ("gap_tol", ctypes.c_double),
+                ("use_graph", ctypes.c_int32)]
+
+
+class _BatchResult(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("lambda_", "primal", "dual", "rel_gap", "nnz_x")] + \
+               [("epochs", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("updates", ctypes.c_uint64)]
+
+
 def _load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -128,6 +141,14 @@ _SIGS = {
     "acdc_sample_coordinates": ([_P, ctypes.c_uint64, ctypes.c_uint64, _I32P], ctypes.c_int),
     "acdc_synchronize": ([_P], ctypes.c_int),
     "acdc_profile_step_kernels": ([_P, ctypes.c_uint64, _F64P, _F64P], ctypes.c_int),
+    "acdc_batch_create": ([ctypes.POINTER(_P), ctypes.c_int32, _F64P, ctypes.c_int32,
+                           ctypes.POINTER(BatchSettings), ctypes.POINTER(_P)], ctypes.c_int),
+    "acdc_batch_run": ([_P, _F64P], ctypes.c_int),
+    "acdc_batch_results": ([_P, ctypes.POINTER(_BatchResult), ctypes.c_int64], ctypes.c_int),
+    "acdc_batch_get_x": ([_P, ctypes.c_int32, _F64P, ctypes.c_int64], ctypes.c_int),
+    "acdc_batch_instance_info": ([_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _F64P,
+                                  _I32P], ctypes.c_int),
+    "acdc_batch_destroy": ([_P], ctypes.c_int),
     "acdc_gen_lasso": ([ctypes.POINTER(GenLassoCfg), ctypes.POINTER(_P)], ctypes.c_int),
     "acdc_gen_svm": ([ctypes.POINTER(GenSvmCfg), ctypes.POINTER(_P)], ctypes.c_int),
     "acdc_gen_ttd": ([ctypes.c_int32, ctypes.POINTER(_P)], ctypes.c_int),
@@ -364,6 +385,65 @@ class Solver:
         self.close()
 
 
+class Batch:
+    """Many Lasso instances, each along a warm-started lambda path, in one
+    persistent launch (acdc_batch_* of the C ABI)."""
+
+    def __init__(self, problems, lambdas, *, seed=1, precision=PREC_FP32_REFRESH,
+                 lanes_per_coord=0, refresh_period=5, max_epochs=200, gap_tol=1e-6,
+                 use_graph=True):
+        lam = _f64(lambdas)
+        if lam.ndim != 2 or lam.shape[0] != len(problems):
+            raise ValueError("lambdas must be (instances, path_len)")
+        self.count, self.path_len = lam.shape
+        self.problems = list(problems)
+        self.n = [p.n for p in problems]
+        arr = (_P * self.count)(*[p._h for p in problems])
+        st = BatchSettings(seed, precision, lanes_per_coord, refresh_period, max_epochs, gap_tol,
+                           int(bool(use_graph)))
+        h = _P()
+        _check(lib.acdc_batch_create(arr, self.count, _ptr(lam, ctypes.c_double), self.path_len,
+                                     ctypes.byref(st), ctypes.byref(h)))
+        self._h = h
+
+    def run(self) -> float:
+        """Solves the whole batch from x = 0; returns the device time in ms."""
+        ms = ctypes.c_double()
+        _check(lib.acdc_batch_run(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def results(self) -> np.ndarray:
+        out = (_BatchResult * (self.count * self.path_len))()
+        _check(lib.acdc_batch_results(self._h, out, self.count * self.path_len))
+        dt = np.dtype([("lambda", "f8"), ("primal", "f8"), ("dual", "f8"), ("rel_gap", "f8"),
+                       ("nnz_x", "f8"), ("epochs", "i4"), ("converged", "i4"), ("updates", "u8")])
+        return np.frombuffer(bytes(out), dtype=dt).reshape(self.count, self.path_len).copy()
+
+    def x(self, i: int) -> np.ndarray:
+        out = np.empty(self.n[i], np.float64)
+        _check(lib.acdc_batch_get_x(self._h, i, _ptr(out, ctypes.c_double), out.size))
+        return out
+
+    def info(self, i: int) -> tuple:
+        tau, beta, lds = ctypes.c_int64(), ctypes.c_double(), ctypes.c_int32()
+        _check(lib.acdc_batch_instance_info(self._h, i, ctypes.byref(tau), ctypes.byref(beta),
+                                            ctypes.byref(lds)))
+        return tau.value, beta.value, bool(lds.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.acdc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def lambda_path(lambda_max: float, path_len: int = 32, ratio: float = 1e-3) -> np.ndarray:
+    """Geometric path from lambda_max down to ratio * lambda_max."""
+    return lambda_max * np.geomspace(1.0, ratio, path_len)
+
+
 # ------------------------------------------------------------- configurations
 def config_lasso_c1(seed: int = 1) -> Dataset:
     """2,000 x 10,000 Lasso, 20 nnz per column, 100 +-1 planted nonzeros,
@@ -385,3 +465,18 @@ def config_svm_c3(seed: int = 3, n_samples: int = 20_000, n_features: int = 1_35
     10,000-sparse w*, 10% label noise; lambda = 1/n."""
     return gen_svm(n_samples, n_features, row_median=300.0, row_sigma=0.901,
                    wstar_nnz=min(10000, n_features), flip_frac=0.1, seed=seed)
+
+
+def config_batch_c5(count: int = 256, seed: int = 5, m_range=(1000, 50000),
+                    n_range=(5000, 200000)):
+    """Batched Lasso workload: instance sizes m ~ logU(m_range), n ~ logU(n_range)
+    (numpy PCG64 keyed by seed), column nnz truncated Zipf(1.1) on [3,100],
+    unit N(0,1) columns, x* with n/100 entries +-1, b = Ax* + 0.01 N(0,1).
+    Instance i is generated with seed 1000*seed + i."""
+    rng = np.random.default_rng(seed)
+    ms = np.exp(rng.uniform(np.log(m_range[0]), np.log(m_range[1]), count)).astype(np.int64)
+    ns = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), count)).astype(np.int64)
+    return [gen_lasso(int(m), int(n), nnz_law=NNZ_ZIPF_TRUNC, nnz_lo=3, nnz_hi=100, zipf_s=1.1,
+                      xstar_nnz=max(1, int(n) // 100), xstar_normal=False, noise=0.01,
+                      lambda_frac=1.0, seed=1000 * seed + i)
+            for i, (m, n) in enumerate(zip(ms, ns))]

@@ -1,0 +1,405 @@
+// batch.hip -- C ABI of the batched multi-instance Lasso solver
+// (acdc_batch_* in include/acdc_hip.h).
+//
+// Layout: all instances' arrays are packed into one allocation per kind
+// (coordinate records, (row, value) entries, FP64 values, b, r64, global dr),
+// and a device table of BatchInstance records points into them.  Instances whose rows fit
+// in 160 KB of LDS form one work list, the rest a second one; each list is a
+// persistent launch of k_batch_lasso over a work-queue counter, largest
+// instances first.  The reset kernel and both launches are captured once in a
+// HIP graph and replayed by acdc_batch_run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/acdc_hip.h"
+#include "problem.h"
+#include "batch_kernels.h"
+
+using namespace acdc;
+
+
+namespace {
+acdc_status bfail(acdc_status st, const std::string &msg) {
+  acdc::set_error(msg.c_str());
+  return st;
+}
+#define BHIP(expr)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return bfail(ACDC_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));     \
+  } while (0)
+
+constexpr size_t kLdsBytes = 160 * 1024 - 1024;  // dynamic LDS per block (static scratch aside)
+// coordinates per lane group and step; with 4 register chunks per column
+// this keeps every instantiation within 128 VGPRs (1024-thread blocks)
+// without scratch spills
+constexpr int kSlotsPerGroup = 2;
+}  // namespace
+
+struct acdc_batch {
+  acdc_batch_settings set{};
+  bool fp32 = true;
+  int lanes = 16;
+  int32_t count = 0, path_len = 0;
+  std::vector<int64_t> n, m, xoff;
+  std::vector<int32_t> tau;
+  std::vector<double> beta;
+  int64_t total_n = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  BatchInstance *dinst = nullptr;
+  CoordRec *dcs = nullptr;    // records of all instances, contiguous
+  int32_t *dorder = nullptr;  // [lds list | global list]
+  int32_t nlds = 0, nglob = 0;
+  double *dlambda = nullptr;
+  unsigned int *dqueue = nullptr;
+  BatchResult *dres = nullptr;
+  int blocks_lds = 0, blocks_glob = 0;
+  double last_ms = 0;
+  std::vector<void *> allocs;
+};
+
+namespace {
+template <typename P>
+acdc_status balloc(acdc_batch *b, P **p, size_t count) {
+  void *v = nullptr;
+  hipError_t e = hipMalloc(&v, std::max<size_t>(count * sizeof(P), 16));
+  if (e != hipSuccess) return bfail(ACDC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  b->allocs.push_back(v);
+  *p = static_cast<P *>(v);
+  return ACDC_OK;
+}
+
+void destroy_batch(acdc_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void *p : b->allocs) (void)hipFree(p);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// column entries cached in registers per lane: 4 chunks, 2 for 64-lane groups
+constexpr int regs_for(int lanes) { return lanes == 64 ? 2 : 4; }
+
+template <typename T, int G, bool LDSR>
+const void *batch_fn() {
+  return reinterpret_cast<const void *>(&k_batch_lasso<T, G, kSlotsPerGroup, regs_for(G), LDSR>);
+}
+
+template <typename T, bool LDSR>
+const void *batch_fn_lanes(int lanes) {
+  switch (lanes) {
+    case 8: return batch_fn<T, 8, LDSR>();
+    case 16: return batch_fn<T, 16, LDSR>();
+    case 32: return batch_fn<T, 32, LDSR>();
+    default: return batch_fn<T, 64, LDSR>();
+  }
+}
+
+const void *batch_kernel(const acdc_batch *b, bool lds) {
+  if (b->fp32) return lds ? batch_fn_lanes<float, true>(b->lanes) : batch_fn_lanes<float, false>(b->lanes);
+  return lds ? batch_fn_lanes<double, true>(b->lanes) : batch_fn_lanes<double, false>(b->lanes);
+}
+
+// enqueue reset + the two persistent launches on b->stream
+acdc_status enqueue_run(acdc_batch *b) {
+  const int64_t nreset = std::max<int64_t>(b->total_n, 2);
+  k_batch_reset<<<(unsigned)((nreset + kBlock - 1) / kBlock), kBlock, 0, b->stream>>>(
+      b->dcs, b->total_n, b->dqueue, 2);
+  BHIP(hipGetLastError());
+  const size_t tsz = b->fp32 ? 4 : 8;
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool lds = pass == 0;
+    const int32_t cnt = lds ? b->nlds : b->nglob;
+    if (cnt == 0) continue;
+    BatchArgs a;
+    a.inst = b->dinst;
+    a.order = b->dorder + (lds ? 0 : b->nlds);
+    a.count = cnt;
+    a.path_len = b->path_len;
+    a.lambdas = b->dlambda;
+    a.refresh_period = b->set.refresh_period;
+    a.max_epochs = b->set.max_epochs;
+    a.gap_tol = b->set.gap_tol;
+    a.queue = b->dqueue + pass;
+    a.results = b->dres;
+    a.lds_doubles = lds ? (int32_t)(kLdsBytes / 8) : 0;
+    void *args[] = {&a};
+    const size_t shmem = lds ? (kLdsBytes / tsz) * tsz : 0;
+    const int blocks = lds ? b->blocks_lds : b->blocks_glob;
+    BHIP(hipLaunchKernel(batch_kernel(b, lds), dim3(std::min(blocks, cnt)), dim3(kBatchBlock), args,
+                         shmem, b->stream));
+  }
+  return ACDC_OK;
+}
+
+template <typename T>
+void pack_entries(const acdc_problem *p, std::vector<Entry<T>> &out) {
+  out.resize(p->nnz);
+  for (int64_t q = 0; q < p->nnz; ++q) {
+    Entry<T> e{};
+    e.row = p->cidx[q];
+    e.val = (T)p->cval[q];
+    out[q] = e;
+  }
+}
+}  // namespace
+
+extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, int32_t count,
+                                         const double *lambdas, int32_t path_len,
+                                         const acdc_batch_settings *set, acdc_batch **out) {
+  if (!problems || !lambdas || !set || !out || count < 1 || path_len < 1)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: null argument or empty batch");
+  if (set->precision != ACDC_PREC_FP64 && set->precision != ACDC_PREC_FP32_REFRESH)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: unknown precision");
+  if (set->refresh_period < 1 || set->max_epochs < 0)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: refresh_period >= 1 and max_epochs >= 0");
+  if (set->lanes_per_coord != 0 && set->lanes_per_coord != 8 && set->lanes_per_coord != 16 &&
+      set->lanes_per_coord != 32 && set->lanes_per_coord != 64)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: lanes_per_coord must be 0, 8, 16, 32 or 64");
+  for (int32_t i = 0; i < count; ++i) {
+    if (!problems[i] || problems[i]->kind != 0)
+      return bfail(ACDC_ERR_INVALID, "acdc_batch_create: every problem must be column-major (Lasso)");
+    for (int32_t l = 0; l < path_len; ++l)
+      if (!(lambdas[(int64_t)i * path_len + l] >= 0))
+        return bfail(ACDC_ERR_INVALID, "acdc_batch_create: lambdas must be >= 0");
+  }
+  int32_t ndev = 0;
+  acdc_device_count(&ndev);
+  if (ndev < 1) return bfail(ACDC_ERR_NO_DEVICE, "acdc_batch_create: no gfx950 device visible");
+
+  acdc_batch *b = new (std::nothrow) acdc_batch();
+  if (!b) return bfail(ACDC_ERR_NOMEM, "acdc_batch_create: out of host memory");
+  auto bail = [&](acdc_status e) {
+    std::string msg = acdc_last_error();
+    destroy_batch(b);
+    acdc::set_error(msg.c_str());
+    return e;
+  };
+  b->set = *set;
+  b->fp32 = set->precision == ACDC_PREC_FP32_REFRESH;
+  b->count = count;
+  b->path_len = path_len;
+  int64_t tot_nnz = 0, tot_n = 0, tot_m = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    tot_nnz += problems[i]->nnz;
+    tot_n += problems[i]->ncoord;
+    tot_m += problems[i]->nres;
+  }
+  b->lanes = set->lanes_per_coord ? set->lanes_per_coord
+                                  : (tot_nnz <= 8 * tot_n ? 8 : (tot_nnz <= 24 * tot_n ? 16 : 32));
+  const int32_t slots = kSlotsPerGroup * (kBatchBlock / b->lanes);
+  const size_t tsz = b->fp32 ? 4 : 8;
+  const size_t esz = b->fp32 ? sizeof(Entry<float>) : sizeof(Entry<double>);
+  const int64_t lds_rows = (int64_t)(kLdsBytes / tsz);
+  b->total_n = tot_n;
+
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return bail(bfail(ACDC_ERR_HIP, "acdc_batch_create: stream/event creation failed"));
+  unsigned char *dent, *ddr = nullptr;
+  double *dval64, *db, *dr64;
+  acdc_status st;
+  if ((st = balloc(b, &b->dcs, tot_n)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &dent, tot_nnz * esz)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &dval64, tot_nnz)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &db, tot_m)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &dr64, tot_m)) != ACDC_OK) return bail(st);
+  int64_t glob_m = 0;
+  for (int32_t i = 0; i < count; ++i)
+    if (problems[i]->nres > lds_rows) glob_m += problems[i]->nres;
+  if (glob_m > 0 && (st = balloc(b, &ddr, glob_m * tsz)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &b->dinst, count)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &b->dorder, count)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &b->dlambda, (size_t)count * path_len)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &b->dqueue, 2)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &b->dres, (size_t)count * path_len)) != ACDC_OK) return bail(st);
+
+  std::vector<BatchInstance> hinst(count);
+  int64_t onnz = 0, on = 0, om = 0, og = 0;
+  hipError_t e = hipSuccess;
+  std::vector<CoordRec> recs;
+  std::vector<Entry<float>> ef;
+  std::vector<Entry<double>> ed;
+  for (int32_t i = 0; i < count && e == hipSuccess; ++i) {
+    const acdc_problem *p = problems[i];
+    BatchInstance &I = hinst[i];
+    I.cs = b->dcs + on;
+    I.ent = dent + onnz * esz;
+    I.val64 = dval64 + onnz;
+    I.b = db + om;
+    I.r64 = dr64 + om;
+    I.drg = nullptr;
+    if (p->nres > lds_rows) {
+      I.drg = ddr + og * tsz;
+      og += p->nres;
+    }
+    I.m = (int32_t)p->nres;
+    I.n = (int32_t)p->ncoord;
+    I.tau = (int32_t)std::min<int64_t>(slots, p->ncoord);
+    I.pad = 0;
+    I.seed = set->seed + (uint64_t)i;
+    const double beta = p->ncoord > 1 ? 1.0 + (double)(p->omega - 1) * (double)(I.tau - 1) /
+                                                  ((double)p->ncoord - 1.0)
+                                      : 1.0;
+    b->n.push_back(p->ncoord);
+    b->m.push_back(p->nres);
+    b->xoff.push_back(on);
+    b->tau.push_back(I.tau);
+    b->beta.push_back(beta);
+    // coordinate records: x = 0, L_j = 1/(beta ||a_j||^2) (loss_square.h:183-194)
+    recs.resize(p->ncoord);
+    for (int64_t j = 0; j < p->ncoord; ++j) {
+      double s2 = 0;
+      for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) s2 += p->cval[q] * p->cval[q];
+      recs[j].x = 0.0;
+      recs[j].L = s2 > 0 ? 1.0 / (beta * s2) : 0.0;
+      recs[j].gref = 0.0;
+      recs[j].p0 = p->cptr[j];
+      recs[j].len = p->cptr[j + 1] - p->cptr[j];
+    }
+    e = hipMemcpy(I.cs, recs.data(), recs.size() * sizeof(CoordRec), hipMemcpyHostToDevice);
+    if (e == hipSuccess && p->nnz) {
+      if (b->fp32) {
+        pack_entries(p, ef);
+        e = hipMemcpy(const_cast<void *>(I.ent), ef.data(), p->nnz * esz, hipMemcpyHostToDevice);
+      } else {
+        pack_entries(p, ed);
+        e = hipMemcpy(const_cast<void *>(I.ent), ed.data(), p->nnz * esz, hipMemcpyHostToDevice);
+      }
+    }
+    if (e == hipSuccess && p->nnz)
+      e = hipMemcpy(const_cast<double *>(I.val64), p->cval.data(), p->nnz * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(const_cast<double *>(I.b), p->rhs.data(), p->nres * 8, hipMemcpyHostToDevice);
+    onnz += p->nnz;
+    on += p->ncoord;
+    om += p->nres;
+  }
+  if (e != hipSuccess)
+    return bail(bfail(ACDC_ERR_HIP, std::string("acdc_batch_create: upload: ") + hipGetErrorString(e)));
+
+  // work lists: LDS-resident rows first, each list largest (nnz) first
+  std::vector<int32_t> ord(count);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) {
+    const bool la = problems[a]->nres <= lds_rows, lc = problems[c]->nres <= lds_rows;
+    if (la != lc) return la;
+    return problems[a]->nnz > problems[c]->nnz;
+  });
+  for (int32_t i : ord) (problems[i]->nres <= lds_rows ? b->nlds : b->nglob)++;
+  BHIP(hipMemcpy(b->dorder, ord.data(), count * 4, hipMemcpyHostToDevice));
+  BHIP(hipMemcpy(b->dinst, hinst.data(), count * sizeof(BatchInstance), hipMemcpyHostToDevice));
+  BHIP(hipMemcpy(b->dlambda, lambdas, (size_t)count * path_len * 8, hipMemcpyHostToDevice));
+
+  // grid sizes: resident blocks of each variant
+  int dev = 0;
+  hipDeviceProp_t prop;
+  BHIP(hipGetDevice(&dev));
+  BHIP(hipGetDeviceProperties(&prop, dev));
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool lds = pass == 0;
+    const size_t shmem = lds ? (kLdsBytes / tsz) * tsz : 0;
+    BHIP(hipFuncSetAttribute(batch_kernel(b, lds), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)shmem));
+    int per_cu = 0;
+    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(b, lds), kBatchBlock,
+                                                      shmem));
+    (lds ? b->blocks_lds : b->blocks_glob) = std::max(1, per_cu) * prop.multiProcessorCount;
+  }
+  BHIP(hipStreamSynchronize(b->stream));
+
+  if (set->use_graph) {
+    BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    st = enqueue_run(b);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(b->stream, &g);
+    if (st != ACDC_OK) return bail(st);
+    if (ce != hipSuccess)
+      return bail(bfail(ACDC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce)));
+    b->graph = g;
+    BHIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  *out = b;
+  return ACDC_OK;
+}
+
+extern "C" acdc_status acdc_batch_run(acdc_batch *b, double *device_ms) {
+  if (!b) return bfail(ACDC_ERR_INVALID, "acdc_batch_run: null batch");
+  BHIP(hipEventRecord(b->ev0, b->stream));
+  if (b->exec) {
+    BHIP(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    acdc_status st = enqueue_run(b);
+    if (st != ACDC_OK) return st;
+  }
+  BHIP(hipEventRecord(b->ev1, b->stream));
+  BHIP(hipEventSynchronize(b->ev1));
+  float ms = 0;
+  BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  if (device_ms) *device_ms = ms;
+  return ACDC_OK;
+}
+
+extern "C" acdc_status acdc_batch_results(acdc_batch *b, acdc_batch_result *out, int64_t len) {
+  if (!b || !out) return bfail(ACDC_ERR_INVALID, "acdc_batch_results: null argument");
+  if (len != (int64_t)b->count * b->path_len)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_results: len must be count * path_len");
+  std::vector<BatchResult> r((size_t)len);
+  BHIP(hipStreamSynchronize(b->stream));
+  BHIP(hipMemcpy(r.data(), b->dres, r.size() * sizeof(BatchResult), hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < len; ++k) {
+    out[k].lambda = r[k].lambda;
+    out[k].primal = r[k].primal;
+    out[k].dual = r[k].dual;
+    out[k].rel_gap = r[k].rel_gap;
+    out[k].nnz_x = r[k].nnz_x;
+    out[k].epochs = r[k].epochs;
+    out[k].converged = r[k].converged;
+    out[k].updates = r[k].updates;
+  }
+  return ACDC_OK;
+}
+
+extern "C" acdc_status acdc_batch_get_x(acdc_batch *b, int32_t instance, double *x, int64_t n) {
+  if (!b || !x) return bfail(ACDC_ERR_INVALID, "acdc_batch_get_x: null argument");
+  if (instance < 0 || instance >= b->count || n != b->n[instance])
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_get_x: bad instance or length");
+  BHIP(hipStreamSynchronize(b->stream));
+  std::vector<CoordRec> recs((size_t)n);
+  BHIP(hipMemcpy(recs.data(), b->dcs + b->xoff[instance], n * sizeof(CoordRec),
+                 hipMemcpyDeviceToHost));
+  for (int64_t j = 0; j < n; ++j) x[j] = recs[j].x;
+  return ACDC_OK;
+}
+
+extern "C" acdc_status acdc_batch_instance_info(const acdc_batch *b, int32_t instance,
+                                                int64_t *tau, double *beta, int32_t *lds_resident) {
+  if (!b || !tau || !beta || !lds_resident)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_instance_info: null argument");
+  if (instance < 0 || instance >= b->count)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_instance_info: bad instance");
+  *tau = b->tau[instance];
+  *beta = b->beta[instance];
+  *lds_resident = b->m[instance] <= (int64_t)(kLdsBytes / (b->fp32 ? 4 : 8));
+  return ACDC_OK;
+}
+
+extern "C" acdc_status acdc_batch_destroy(acdc_batch *b) {
+  destroy_batch(b);
+  return ACDC_OK;
+}

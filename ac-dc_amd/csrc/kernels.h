@@ -316,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_lipschitz(Seg<T> A, int64_t nseg, do
 // MulticoreSquareLoss::recomputeResiduals, MulticoreSquareLoss.h:42-53, and of
 // MulticoreSVMDualLoss.h:31-46 with alpha clipped to [0,1]).  Only segments
 // of nonzero coordinates are read, so a sparse x costs O(n + nnz(active)).
-__global__ __launch_bounds__(kBlock) void k_res_init(int64_t nres, const double *__restrict__ b,
+static __global__ __launch_bounds__(kBlock) void k_res_init(int64_t nres, const double *__restrict__ b,
                                                      double *__restrict__ r64) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k < nres) r64[k] = b ? -b[k] : 0.0;
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_coord(Seg<T> A, int64_t nseg,
 }
 
 // Per-residual-entry statistics: {sum r_k^2, sum b_k r_k}.
-__global__ __launch_bounds__(kBlock) void k_eval_res(int64_t nres, const double *__restrict__ r64,
+static __global__ __launch_bounds__(kBlock) void k_eval_res(int64_t nres, const double *__restrict__ r64,
                                                      const double *__restrict__ b,
                                                      double *__restrict__ partial) {
   double v0 = 0, v1 = 0;
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_res(int64_t nres, const double 
 
 // Folds nrows partial rows into out[kStats] with one block: thread t folds
 // rows t, t+256, ... in order, then a fixed pairwise tree.
-__global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
+static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
                                                         int64_t nrows, uint32_t maxmask,
                                                         uint32_t minmask,
                                                         double *__restrict__ out) {
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict
 
 // --------------------------------------------------------------- sampling
 // out[s*tau + i] = perm_{seed, step0+s}(i): the device side of the host replay.
-__global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t seed, uint64_t step0,
+static __global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t seed, uint64_t step0,
                                                    uint64_t nsteps, int64_t tau,
                                                    int32_t *__restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/acdc_hip.h"
+#include "problem.h"
 #include "kernels.h"
 
 using namespace acdc;
@@ -64,17 +65,6 @@ constexpr uint64_t kStepsPerLaunch = 4096;
 constexpr int kRegChunks = 2;
 }  // namespace
 
-// ----------------------------------------------------------------- problem
-struct acdc_problem {
-  int kind;        // 0: column-major input (Lasso/NNLS), 1: sample rows (SVM)
-  int64_t ncoord;  // coordinates: columns n, or samples
-  int64_t nres;    // residual entries: rows m, or features
-  int64_t nnz;
-  std::vector<int32_t> cptr, cidx;  // coordinate-major segments
-  std::vector<double> cval;
-  std::vector<double> rhs;  // b (m) or labels y (samples)
-  int32_t omega;
-};
 
 namespace {
 acdc_status check_segments(int64_t nseg, int64_t nidx, int64_t nnz, const int32_t *ptr,

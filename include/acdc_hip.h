@@ -198,6 +198,48 @@ acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps,
 /* Waits for all work queued on the solver's stream. */
 acdc_status acdc_synchronize(acdc_solver *s);
 
+/* ---- batched Lasso: many instances x warm-started lambda paths ----------
+ * One persistent launch (two when some instances have more rows than the
+ * LDS residual holds) solves every instance along its lambda path: lambda l+1
+ * starts from the x of lambda l (lasso_console_solver.cpp:149-172), and each
+ * (instance, lambda) stops at FP64 relative duality gap <= gap_tol (checked
+ * with an FP64 refresh every refresh_period epochs, mSDCA.cpp:123 stops on
+ * the gap the same way) or after max_epochs.  Instances are handed to
+ * workgroups by a device work queue, largest first.  Instance i samples
+ * with key seed + i and tau = 4 * (1024 / lanes_per_coord) coordinates per
+ * step (capped at its n); its ESO beta comes from its omega.  No counterpart
+ * in the reference beyond the lambda-path driver it generalises. */
+typedef struct acdc_batch acdc_batch;
+
+typedef struct acdc_batch_settings {
+  uint64_t seed;
+  int32_t precision;       /* ACDC_PREC_FP32_REFRESH or ACDC_PREC_FP64 */
+  int32_t lanes_per_coord; /* 0 = auto, else 8, 16, 32 or 64 */
+  int32_t refresh_period;  /* epochs between FP64 refresh + gap checks (>= 1) */
+  int32_t max_epochs;      /* per lambda */
+  double gap_tol;          /* relative duality gap target (< 0: always max_epochs) */
+  int32_t use_graph;       /* capture the run in a HIP graph at create */
+} acdc_batch_settings;
+
+typedef struct acdc_batch_result {
+  double lambda, primal, dual, rel_gap, nnz_x;
+  int32_t epochs, converged;
+  uint64_t updates; /* coordinate updates of the instance up to this lambda */
+} acdc_batch_result;
+
+acdc_status acdc_batch_create(const acdc_problem *const *problems, int32_t count,
+                              const double *lambdas /* count x path_len */,
+                              int32_t path_len, const acdc_batch_settings *s,
+                              acdc_batch **out);
+/* Runs the whole batch from x = 0; *device_ms (may be null) gets the device
+ * time of the run. */
+acdc_status acdc_batch_run(acdc_batch *b, double *device_ms);
+acdc_status acdc_batch_results(acdc_batch *b, acdc_batch_result *out, int64_t len);
+acdc_status acdc_batch_get_x(acdc_batch *b, int32_t instance, double *x, int64_t n);
+acdc_status acdc_batch_instance_info(const acdc_batch *b, int32_t instance, int64_t *tau,
+                                     double *beta, int32_t *lds_resident);
+acdc_status acdc_batch_destroy(acdc_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

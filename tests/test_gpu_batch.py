@@ -1,0 +1,100 @@
+"""Batched multi-instance Lasso (acdc_batch_*) against the FP64 oracle.
+
+Instance i of a batch samples with key seed + i and its own tau and ESO beta,
+so its trajectory is the oracle's synchronous run with those parameters; the
+lambda path is warm-started, so the oracle carries x from one lambda to the
+next.  Covers both residual placements (LDS for short instances, global for
+one taller than the LDS holds), FP64 (tight tolerance) and FP32 + anchored
+refresh (converges to the FP64 optimum), and graph replays against an eager
+run.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch(acdc):
+    shapes = [(300, 1000, 6), (1200, 2500, 10), (800, 4000, 4), (2500, 1500, 20),
+              (45000, 2000, 30)]  # the last one does not fit the LDS residual
+    ds = []
+    for i, (m, n, k) in enumerate(shapes):
+        ds.append(acdc.gen_lasso(m, n, nnz_law=acdc.NNZ_FIXED, nnz_fixed=k,
+                                 xstar_nnz=max(1, n // 100), noise=0.01, lambda_frac=1.0,
+                                 seed=100 + i))
+    return ds
+
+
+def lambdas_of(acdc, ds, path_len, ratio):
+    return np.stack([acdc.lambda_path(d.lambda_max, path_len, ratio) for d in ds])
+
+
+def oracle_path(orc, d, lams, seed, tau, beta, epochs):
+    o = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[0], beta=beta)
+    steps = epochs * (-(-d.n // tau))
+    step0 = 0
+    for lam in lams:
+        o.lam = float(lam)
+        o.run(seed=seed, tau=tau, step0=step0, nsteps=steps)
+        step0 += steps
+    return o
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 32])
+def test_batch_fp64_fixed_epochs_matches_oracle(acdc, orc, small_batch, lanes):
+    ds = small_batch
+    lams = lambdas_of(acdc, ds, 3, 0.1)
+    probs = [d.problem() for d in ds]
+    b = acdc.Batch(probs, lams, seed=40, precision=acdc.PREC_FP64, lanes_per_coord=lanes,
+                   refresh_period=1, max_epochs=2, gap_tol=-1.0, use_graph=False)
+    b.run()
+    res = b.results()
+    assert (res["epochs"] == 2).all()
+    for i, d in enumerate(ds):
+        tau, beta, lds = b.info(i)
+        assert lds == (d.m <= 19968)
+        o = oracle_path(orc, d, lams[i], 40 + i, tau, beta, 2)
+        x = b.x(i)
+        err = np.max(np.abs(x - o.x)) / max(1.0, np.max(np.abs(o.x)))
+        assert err < 1e-10, (i, err)
+        e = o.eval()
+        assert res[i, -1]["primal"] == pytest.approx(e["primal"], rel=1e-10)
+        assert res[i, -1]["rel_gap"] == pytest.approx(e["rel_gap"], rel=1e-6, abs=1e-12)
+    b.close()
+
+
+def test_batch_fp32_converges_to_fp64_optimum(acdc, orc, small_batch):
+    ds = small_batch
+    lams = lambdas_of(acdc, ds, 4, 0.05)
+    probs = [d.problem() for d in ds]
+    b = acdc.Batch(probs, lams, seed=7, precision=acdc.PREC_FP32_REFRESH, refresh_period=5,
+                   max_epochs=400, gap_tol=1e-7, use_graph=True)
+    b.run()
+    res = b.results()
+    assert res["converged"].all(), res["rel_gap"]
+    for i, d in enumerate(ds):
+        o = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[i, -1])
+        e = o.eval(b.x(i))
+        # the FP64 oracle agrees with the device's own FP64 evaluation
+        assert e["rel_gap"] <= 1.01e-7
+        assert res[i, -1]["primal"] == pytest.approx(e["primal"], rel=1e-9)
+    b.close()
+
+
+def test_batch_graph_replays_match_eager(acdc, small_batch):
+    ds = small_batch
+    lams = lambdas_of(acdc, ds, 3, 0.1)
+    probs = [d.problem() for d in ds]
+    kw = dict(seed=3, precision=acdc.PREC_FP64, refresh_period=2, max_epochs=4, gap_tol=-1.0)
+    eager = acdc.Batch(probs, lams, use_graph=False, **kw)
+    eager.run()
+    ref = [eager.x(i) for i in range(len(ds))]
+    g = acdc.Batch(probs, lams, use_graph=True, **kw)
+    for rep in range(3):
+        g.run()
+        for i in range(len(ds)):
+            x = g.x(i)  # eager host work and a synchronisation between replays
+            assert np.max(np.abs(x - ref[i])) <= 1e-10 * max(1.0, np.max(np.abs(ref[i]))), (rep, i)
+    eager.close()
+    g.close()
