@@ -33,6 +33,7 @@ BETA_OMEGA = 0
 BETA_FORCED = 1
 BETA_NONE = 2
 STEP_SYNC = 0
+STEP_ASYNC = 1
 LAUNCH_AUTO = 0
 LAUNCH_PER_STEP = 1
 LAUNCH_PERSISTENT = 2

@@ -152,7 +152,8 @@ struct ColCache {
   __device__ __forceinline__ V dot(const Entry<T> *ent, const V *v, int lane) const {
     V acc = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc += (V)val[r] * v[row[r]];
+    for (int r = 0; r < R; ++r)  // padding lanes read nothing
+      if (lane + r * G < len) acc += (V)val[r] * v[row[r]];
     for (int32_t q = lane + R * G; q < len; q += G) {
       const Entry<T> e = ent[p0 + q];
       acc += (V)e.val * v[e.row];

@@ -65,6 +65,23 @@ ACDC_HD StepKeys step_keys(uint64_t seed, uint64_t step) {
   return s;
 }
 
+// Asynchronous workers sample with replacement, each from its own stream,
+// like the reference's per-thread generators (randomNumbersUtil.h:83-125
+// drawn by gsl_rng_uniform_int, MulticoreEngineExecutor.h:86).  Update k of
+// worker w takes 64 bits of Philox4x32-10 at counter (k/2, w, "PCDA") --
+// two updates per block of output -- and maps them to [0, n) by the high
+// word of the 64 x 32-bit product (multiply-shift; bias below n / 2^64).
+constexpr uint32_t kAsyncTag = 0x50434441u;  // "PCDA"
+
+ACDC_HD uint32_t async_coord(uint64_t seed, uint32_t worker, uint64_t k, uint32_t n) {
+  const uint64_t blk = k >> 1;
+  uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), worker, kAsyncTag};
+  Philox::gen(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint32_t lo = (k & 1) ? c[2] : c[0], hi = (k & 1) ? c[3] : c[1];
+  // floor((hi 2^32 + lo) n / 2^64), exact in 64-bit pieces
+  return (uint32_t)(((uint64_t)hi * n + (((uint64_t)lo * n) >> 32)) >> 32);
+}
+
 // Feistel round function: a 32-bit multiply/xor-shift integer mixer.
 ACDC_HD uint32_t mix32(uint32_t x, uint32_t k) {
   uint32_t h = x ^ k;

@@ -236,7 +236,8 @@ __global__ __launch_bounds__(kPersistentBlock) void k_sync_persistent(
     if (active) {
       T acc = 0;
 #pragma unroll
-      for (int r = 0; r < R; ++r) acc += cv[r] * ld_coherent(&a.res[ci[r]]);
+      for (int r = 0; r < R; ++r)  // padding lanes issue no load
+        if (p0 + lane + r * G < p1) acc += cv[r] * ld_coherent(&a.res[ci[r]]);
       for (int32_t p = p0 + lane + R * G; p < p1; p += G)
         acc += a.A.val[p] * ld_coherent(&a.res[a.A.idx[p]]);
       acc = group_sum<G>(acc);
@@ -288,6 +289,118 @@ __global__ __launch_bounds__(64) void k_serial(StepArgs<T> a, uint64_t step0, ui
                                __HIP_MEMORY_SCOPE_AGENT);
     }
     __threadfence();
+  }
+}
+
+// ----------------------------------------------------- asynchronous workers
+// tau workers, one G-lane group each, run `count` updates apiece with no
+// synchronisation between them: the reference's multicore executor
+// (MulticoreEngineExecutor::executeBulkOfIterations, :83-89; the update
+// MulticoreL1RegSquareLoss.h:53-63 with parallel::atomic_add,
+// parallel_essentials_posix.h:62-68) and its CUDA twin
+// (randomL2L1Solver.h:15-62).  Residual and x are read with agent-scope
+// loads, so a worker sees every update that has reached the device-coherent
+// level, and written with hardware atomic adds (x_j += d, r += c a_j), so
+// no update is lost.  At most tau updates are in flight at once.
+//
+// Pipelining: everything an update needs apart from the residual and x_j --
+// its coordinate, segment bounds, L_j, the anchored gradient, the label and
+// the first R*G segment entries -- is loaded one update ahead (the entries)
+// or two ahead (the rest), into two alternating register sets.  Those
+// prefetches are issued just before the update's own gather, so a single
+// wait per update covers every load in flight (vector memory counters drain
+// in order, so waiting for a gather issued before the prefetches would wait
+// for them anyway) and the critical path of an update is one round trip.
+// With TAIL = false every segment fits the R*G cached entries (the host
+// checks the longest), which keeps variable-length loops out of the update.
+// With tau = 1 the worker also waits for its scatter before the next gather,
+// which makes the run the serial method exactly.
+constexpr int kAsyncBlock = 256;
+
+template <typename T, int R>
+struct AsyncCoord {
+  uint32_t j;
+  int32_t p0, p1;
+  double L, gref, y;
+  int32_t ci[R];
+  T cv[R];
+};
+
+template <typename T, int G, int LOSS, int R, bool TAIL>
+__global__ __launch_bounds__(kAsyncBlock) void k_async(StepArgs<T> a, uint64_t k0,
+                                                       uint64_t count) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t w = ((int64_t)blockIdx.x * kAsyncBlock + threadIdx.x) / G;
+  if (w >= a.tau || count == 0) return;  // whole groups leave together
+  const uint32_t n = a.perm.n;
+  const bool serial = a.tau == 1;
+  // stage 1: coordinate and per-coordinate scalars of update k
+  auto bounds = [&](uint64_t k, AsyncCoord<T, R> &c) {
+    c.j = async_coord(a.seed, (uint32_t)w, k0 + k, n);
+    c.p0 = a.A.ptr[c.j];
+    c.p1 = a.A.ptr[c.j + 1];
+    c.L = a.L[c.j];
+    c.gref = a.gref ? a.gref[c.j] : 0.0;
+    c.y = LOSS == kSvm ? a.y[c.j] : 0.0;
+  };
+  // stage 2: the cached segment entries
+  auto entries = [&](AsyncCoord<T, R> &c) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t p = c.p0 + lane + r * G;
+      c.ci[r] = p < c.p1 ? a.A.idx[p] : 0;
+      c.cv[r] = p < c.p1 ? a.A.val[p] : (T)0;
+    }
+  };
+  // update k on cur; meanwhile nxt (update k+1) gets its entries and the
+  // scalars of update k+2 land in ahead, which becomes cur's next content
+  auto update = [&](uint64_t k, AsyncCoord<T, R> &cur, AsyncCoord<T, R> &nxt,
+                    AsyncCoord<T, R> &ahead) {
+    // prefetches first, then the gather: every load of the update is in
+    // flight together and one wait covers them (and the previous scatter)
+    if (k + 1 < count) entries(nxt);
+    if (k + 2 < count) bounds(k + 2, ahead);
+    T v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)  // padding lanes issue no load
+      v[r] = cur.p0 + lane + r * G < cur.p1 ? ld_coherent(&a.res[cur.ci[r]]) : (T)0;
+    const double xj = ld_coherent(&a.x[cur.j]);
+    T acc = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc += cur.cv[r] * v[r];
+    if (TAIL)
+      for (int32_t p = cur.p0 + lane + R * G; p < cur.p1; p += G)
+        acc += a.A.val[p] * ld_coherent(&a.res[a.A.idx[p]]);
+    acc = group_sum<G>(acc);
+    double c;
+    const double g = a.gref ? cur.gref + (double)acc : (double)acc;
+    const double d = prox_delta<LOSS>(g, xj, cur.L, cur.y, a.sc, &c);
+    if (lane == 0 && d != 0.0) unsafeAtomicAdd(&a.x[cur.j], d);
+    if (c != 0.0) {
+      const T ct = (T)c;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (cur.p0 + lane + r * G < cur.p1) unsafeAtomicAdd(&a.res[cur.ci[r]], ct * cur.cv[r]);
+      if (TAIL)
+        for (int32_t p = cur.p0 + lane + R * G; p < cur.p1; p += G)
+          unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
+    }
+    if (serial) wait_memory();
+    // the scalars of update k+2 move into cur, whose update is finished
+    cur.j = ahead.j;
+    cur.p0 = ahead.p0;
+    cur.p1 = ahead.p1;
+    cur.L = ahead.L;
+    cur.gref = ahead.gref;
+    cur.y = ahead.y;
+  };
+  AsyncCoord<T, R> s0, s1, ahead;
+  bounds(0, s0);
+  entries(s0);
+  if (count > 1) bounds(1, s1);
+  for (uint64_t k = 0; k < count; k += 2) {
+    update(k, s0, s1, ahead);
+    if (k + 1 < count) update(k + 1, s1, s0, ahead);
   }
 }
 
@@ -500,6 +613,17 @@ static __global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t se
   const uint64_t s = (uint64_t)(t / tau);
   const uint32_t i = (uint32_t)(t % tau);
   out[t] = (int32_t)perm(i, step_keys(seed, step0 + s));
+}
+
+// out[s*tau + w] = coordinate of update step0+s of asynchronous worker w.
+static __global__ __launch_bounds__(kBlock) void k_sample_async(uint32_t n, uint64_t seed,
+                                                         uint64_t step0, uint64_t nsteps,
+                                                         int64_t tau, int32_t *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)nsteps * tau) return;
+  const uint64_t s = (uint64_t)(t / tau);
+  const uint32_t w = (uint32_t)(t % tau);
+  out[t] = (int32_t)async_coord(seed, w, step0 + s, n);
 }
 
 // v *= f (the SVM's w, its anchor and margins scale with 1/lambda).

@@ -61,6 +61,10 @@ constexpr unsigned int kSpinLimit = 1u << 22;
 constexpr int64_t kEvalBlocks = 8192;
 // steps per persistent launch
 constexpr uint64_t kStepsPerLaunch = 4096;
+// updates per asynchronous worker and launch (bounds one kernel's duration)
+constexpr uint64_t kAsyncPerLaunch = 1024;
+// segment entries an async worker caches per lane
+constexpr int kAsyncChunks = 4;
 // segment entries per lane kept in registers by the persistent kernel
 constexpr int kRegChunks = 2;
 }  // namespace
@@ -181,6 +185,8 @@ struct acdc_solver {
   int eval_lanes = 64;  // lanes per coordinate in refresh/evaluation
   bool fp32 = false;
   bool persistent = false;
+  bool async = false;  // ACDC_STEP_ASYNC
+  int64_t maxlen = 0;  // longest segment
   int64_t coresident_blocks = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -289,6 +295,20 @@ struct LaunchStep {
     const StepArgs<T> a = step_args<T>(s);
     if (part & 1) k_step_gather<T, G, LOSS><<<blocks, kBlock, 0, s->stream>>>(a, step);
     if (part & 2) k_step_scatter<T, G><<<blocks, kBlock, 0, s->stream>>>(a);
+  }
+};
+
+// async workers: count updates each, from per-worker update k0 on
+template <typename T, int G, int LOSS>
+struct LaunchAsync {
+  static void run(const acdc_solver *s, uint64_t k0, uint64_t count) {
+    const unsigned blocks = (unsigned)div_up(s->set.tau * G, kAsyncBlock);
+    if (s->maxlen <= kAsyncChunks * G)
+      k_async<T, G, LOSS, kAsyncChunks, false>
+          <<<blocks, kAsyncBlock, 0, s->stream>>>(step_args<T>(s), k0, count);
+    else
+      k_async<T, G, LOSS, kAsyncChunks, true>
+          <<<blocks, kAsyncBlock, 0, s->stream>>>(step_args<T>(s), k0, count);
   }
 };
 
@@ -428,7 +448,7 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: tau must lie in [1, n]");
   if (set->precision != ACDC_PREC_FP64 && set->precision != ACDC_PREC_FP32_REFRESH)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown precision");
-  if (set->step_mode != ACDC_STEP_SYNC)
+  if (set->step_mode != ACDC_STEP_SYNC && set->step_mode != ACDC_STEP_ASYNC)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown step mode");
   if (set->launch_mode < ACDC_LAUNCH_AUTO || set->launch_mode > ACDC_LAUNCH_PERSISTENT)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown launch mode");
@@ -524,7 +544,10 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     return bail(fail(ACDC_ERR_HIP, std::string("acdc_solver_create: upload: ") + hipGetErrorString(e)));
 
   // persistent grid: every slot's group must be resident at once
-  if (set->tau >= 2 && set->launch_mode != ACDC_LAUNCH_PER_STEP) {
+  s->async = set->step_mode == ACDC_STEP_ASYNC;
+  for (size_t c = 0; c + 1 < p->cptr.size(); ++c)
+    s->maxlen = std::max<int64_t>(s->maxlen, (int64_t)p->cptr[c + 1] - p->cptr[c]);
+  if (!s->async && set->tau >= 2 && set->launch_mode != ACDC_LAUNCH_PER_STEP) {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     (void)hipGetDevice(&dev);
@@ -568,7 +591,10 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
       const uint64_t left = period > s->updates_since_refresh ? period - s->updates_since_refresh : 1;
       chunk = std::min(chunk, (left + tau - 1) / tau);
     }
-    if (tau == 1) {
+    if (s->async) {
+      for (uint64_t k = 0; k < chunk; k += kAsyncPerLaunch)
+        dispatch3<LaunchAsync>(s, s->lanes, s->step + k, std::min(kAsyncPerLaunch, chunk - k));
+    } else if (tau == 1) {
       dispatch3<LaunchSerial>(s, s->lanes, s->step, chunk);
     } else if (s->persistent) {
       if ((st = run_persistent(s, s->step, chunk)) != ACDC_OK) return st;
@@ -737,8 +763,12 @@ extern "C" acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0, u
     return fail(ACDC_ERR_INVALID, "acdc_sample_coordinates: at most 2^30 samples per call");
   int32_t *d = nullptr;
   HIP_TRY(hipMalloc(&d, (size_t)total * 4));
-  k_sample<<<(unsigned)div_up(total, kBlock), kBlock, 0, s->stream>>>(
-      Perm::make((uint32_t)s->ncoord), s->set.seed, step0, nsteps, s->set.tau, d);
+  if (s->async)
+    k_sample_async<<<(unsigned)div_up(total, kBlock), kBlock, 0, s->stream>>>(
+        (uint32_t)s->ncoord, s->set.seed, step0, nsteps, s->set.tau, d);
+  else
+    k_sample<<<(unsigned)div_up(total, kBlock), kBlock, 0, s->stream>>>(
+        Perm::make((uint32_t)s->ncoord), s->set.seed, step0, nsteps, s->set.tau, d);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)total * 4, hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -758,8 +788,9 @@ extern "C" acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps
                                                  double *scatter_ms) {
   if (!s || !gather_ms || !scatter_ms)
     return fail(ACDC_ERR_INVALID, "acdc_profile_step_kernels: null argument");
-  if (s->set.tau < 2)
-    return fail(ACDC_ERR_STATE, "acdc_profile_step_kernels: needs tau >= 2 (two-phase steps)");
+  if (s->set.tau < 2 || s->async)
+    return fail(ACDC_ERR_STATE,
+                "acdc_profile_step_kernels: needs synchronous steps with tau >= 2 (two phases)");
   if (nsteps < 1 || nsteps > 4096)
     return fail(ACDC_ERR_INVALID, "acdc_profile_step_kernels: nsteps must lie in [1, 4096]");
   std::vector<hipEvent_t> ev(3 * nsteps, nullptr);

@@ -175,10 +175,10 @@ def main():
         if x32 is not None:
             result["time_to_gap"]["rel_gap_fp64_oracle"] = o.eval(x32)["rel_gap"]
         t0 = time.perf_counter()
-        done = 0
+        done, per = 0, max(1, n // 10 // threads)
         while time.perf_counter() - t0 < args.cpu_seconds and done < 30 * n:
-            o.run_async(seed=2, update0=done, n_updates=n // 10, threads=threads)
-            done += n // 10
+            o.run_async(seed=2, k0=done // threads, per_worker=per, threads=threads)
+            done += per * threads
         cw = time.perf_counter() - t0
         result["cpu_baseline"] = {"value": done / cw, "unit": "coord updates/s", "cores": threads,
                                   "kind": "port",

@@ -55,9 +55,15 @@ typedef enum acdc_beta_mode {
 } acdc_beta_mode;
 
 typedef enum acdc_step_mode {
-  ACDC_STEP_SYNC = 0 /* tau distinct coordinates per step, all deltas from
-                        one state then applied (cdn_common.h:235-282);
-                        tau = 1 is the serial method (loss_square.h:72-83) */
+  ACDC_STEP_SYNC = 0, /* tau distinct coordinates per step, all deltas from
+                         one state then applied (cdn_common.h:235-282);
+                         tau = 1 is the serial method (loss_square.h:72-83) */
+  ACDC_STEP_ASYNC = 1 /* tau unsynchronised workers, each drawing coordinates
+                         with replacement from its own stream and applying
+                         its updates with atomic adds: the multicore executor
+                         (MulticoreEngineExecutor.h:83-89 with
+                         parallel_essentials_posix.h:62-68).  A "step" is one
+                         update per worker; tau = 1 is the serial method. */
 } acdc_step_mode;
 
 /* Solver settings; the reference counterpart is OptimizationSettings
@@ -183,7 +189,8 @@ acdc_status acdc_set_lambda(acdc_solver *s, double lambda);
 acdc_status acdc_get_beta(const acdc_solver *s, double *beta);
 
 /* Coordinates the device samples for steps [step0, step0+nsteps): out has
- * nsteps*tau entries, row-major.  Used to replay the stream on the host. */
+ * nsteps*tau entries, row-major (ACDC_STEP_ASYNC: entry [s][w] is update
+ * step0+s of worker w).  Used to replay the stream on the host. */
 acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0,
                                     uint64_t nsteps, int32_t *out);
 

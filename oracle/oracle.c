@@ -20,7 +20,13 @@
  *                       closed form gap(0) = ||b||^2 (1-s)^2/2.
  *   orc_svm_*           scipy L-BFGS-B on the box-constrained dual (golden),
  *                       brute force for n <= 8, P >= D, w identity.
- *   orc_cd_async_omp    1 thread == orc_cd_run(tau=1) bit for bit.
+ *   orc_async_coords    multiply-shift map checked against exact integer
+ *                       arithmetic; chi^2 uniformity; per-worker streams
+ *                       distinct.
+ *   orc_cd_list         equals orc_cd_run(tau=1) on the coordinates that
+ *                       orc_sample_steps gives it (bit for bit).
+ *   orc_cd_async_omp    1 thread == orc_cd_list over orc_async_coords, bit
+ *                       for bit; many threads converge to the serial optimum.
  *
  * Compiled with -ffp-contract=off so that every a*b+c rounds twice, like the
  * reference's plain C++ loops.
@@ -409,37 +415,72 @@ static double racy_load(const double *a) {
   return d;
 }
 
+/* update k of asynchronous worker w: 64 bits of Philox at counter
+ * (k/2, w, "PCDA") mapped to [0,n) by the high word of the product */
+#define ASYNC_TAG 0x50434441u
+
+static uint32_t async_coord(uint64_t seed, uint32_t w, uint64_t k, uint32_t n) {
+  uint64_t blk = k >> 1;
+  uint32_t ctr[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), w, ASYNC_TAG};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t o[4];
+  orc_philox4x32_10(ctr, key, o);
+  uint32_t lo = (k & 1) ? o[2] : o[0], hi = (k & 1) ? o[3] : o[1];
+  /* floor((hi 2^32 + lo) n / 2^64) with a 128-bit product */
+  unsigned __int128 u = ((unsigned __int128)(((uint64_t)hi << 32) | lo)) * n;
+  return (uint32_t)(u >> 64);
+}
+
+void orc_async_coords(uint64_t seed, uint32_t worker, uint64_t k0, uint64_t count,
+                      uint32_t n, int32_t *out) {
+  for (uint64_t k = 0; k < count; ++k)
+    out[k] = (int32_t)async_coord(seed, worker, k0 + k, n);
+}
+
+void orc_cd_list(int loss, int64_t m, int64_t n, const int32_t *col_ptr,
+                 const int32_t *row_idx, const double *vals, const double *L,
+                 double lambda, const int32_t *coords, uint64_t count,
+                 double *x, double *r) {
+  (void)m;
+  (void)n;
+  for (uint64_t k = 0; k < count; ++k) {
+    int32_t j = coords[k];
+    double d = cd_delta(loss, col_ptr, row_idx, vals, L, lambda, x, r, j);
+    x[j] += d;
+    for (int32_t p = col_ptr[j]; p < col_ptr[j + 1]; ++p)
+      r[row_idx[p]] += d * vals[p];
+  }
+}
+
 int orc_cd_async_omp(int loss, int64_t m, int64_t n, const int32_t *col_ptr,
                      const int32_t *row_idx, const double *vals,
                      const double *L, double lambda, uint64_t seed,
-                     uint64_t update0, uint64_t n_updates, int threads,
+                     uint64_t k0, uint64_t per_worker, int threads,
                      double *x, double *r) {
   (void)m;
   if (threads < 1) threads = 1;
-  uint64_t T = (uint64_t)threads;
 #ifdef _OPENMP
   omp_set_num_threads(threads);
 #endif
-#pragma omp parallel for schedule(static)
-  for (int64_t uu = 0; uu < (int64_t)n_updates; ++uu) {
-    uint64_t u = update0 + (uint64_t)uu;
-    uint32_t k[4];
-    orc_step_keys(seed, u / T, k);
-    int32_t j = (int32_t)orc_perm((uint32_t)(u % T), (uint32_t)n, k);
-    double g = 0;
-    for (int32_t p = col_ptr[j]; p < col_ptr[j + 1]; ++p)
-      g += vals[p] * racy_load(&r[row_idx[p]]);
-    double xj = racy_load(&x[j]);
-    double d;
-    if (loss == 10) {
-      double v = xj - L[j] * g;
-      d = (v > 0 ? v : 0.0) - xj;
-    } else {
-      d = orc_soft_threshold(L[j] * lambda, xj - L[j] * g) - xj;
+#pragma omp parallel for schedule(static, 1)
+  for (int w = 0; w < threads; ++w) {
+    for (uint64_t k = 0; k < per_worker; ++k) {
+      int32_t j = (int32_t)async_coord(seed, (uint32_t)w, k0 + k, (uint32_t)n);
+      double g = 0;
+      for (int32_t p = col_ptr[j]; p < col_ptr[j + 1]; ++p)
+        g += vals[p] * racy_load(&r[row_idx[p]]);
+      double xj = racy_load(&x[j]);
+      double d;
+      if (loss == 10) {
+        double v = xj - L[j] * g;
+        d = (v > 0 ? v : 0.0) - xj;
+      } else {
+        d = orc_soft_threshold(L[j] * lambda, xj - L[j] * g) - xj;
+      }
+      cas_add(&x[j], d);
+      for (int32_t p = col_ptr[j]; p < col_ptr[j + 1]; ++p)
+        cas_add(&r[row_idx[p]], d * vals[p]);
     }
-    cas_add(&x[j], d);
-    for (int32_t p = col_ptr[j]; p < col_ptr[j + 1]; ++p)
-      cas_add(&r[row_idx[p]], d * vals[p]);
   }
   return threads;
 }

@@ -99,15 +99,27 @@ void orc_svm_eval(int64_t ns, int64_t nf, const int32_t *row_ptr,
                   const int32_t *col_idx, const double *vals, const double *y,
                   double lambda, const double *alpha, double out[8]);
 
-/* ---- asynchronous OpenMP restatement (CPU baseline) --------------------
- * MulticoreEngineExecutor::executeBulkOfIterations, :83-89: an OpenMP
- * parallel for over n_updates, unsynchronised reads of x and r, CAS-loop
- * atomic adds (parallel_essentials_posix.h:62-68).  Coordinates come from
- * the same sampler with tau = threads.  Returns the number of threads. */
+/* ---- asynchronous workers -------------------------------------------- */
+/* Coordinates of updates [k0, k0+count) of asynchronous worker `worker`:
+ * with replacement, one Philox4x32-10 stream per worker (the reference's
+ * per-thread generators drawn by gsl_rng_uniform_int,
+ * MulticoreEngineExecutor.h:86, randomNumbersUtil.h:83-125). */
+void orc_async_coords(uint64_t seed, uint32_t worker, uint64_t k0, uint64_t count,
+                      uint32_t n, int32_t *out);
+/* serial updates over the given coordinates, in order (the tau = 1 method,
+ * loss_square.h:72-83; loss 0 soft threshold, 10 clip at 0) */
+void orc_cd_list(int loss, int64_t m, int64_t n, const int32_t *col_ptr,
+                 const int32_t *row_idx, const double *vals, const double *L,
+                 double lambda, const int32_t *coords, uint64_t count,
+                 double *x, double *r);
+/* MulticoreEngineExecutor::executeBulkOfIterations, :83-89: `threads` OpenMP
+ * workers, each running updates [k0, k0+per_worker) of its own stream with
+ * unsynchronised reads of x and r and CAS-loop atomic adds
+ * (parallel_essentials_posix.h:62-68).  Returns the number of threads. */
 int orc_cd_async_omp(int loss, int64_t m, int64_t n, const int32_t *col_ptr,
                      const int32_t *row_idx, const double *vals,
                      const double *L, double lambda, uint64_t seed,
-                     uint64_t update0, uint64_t n_updates, int threads,
+                     uint64_t k0, uint64_t per_worker, int threads,
                      double *x, double *r);
 
 #ifdef __cplusplus

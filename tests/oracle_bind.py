@@ -48,6 +48,9 @@ def lib() -> ctypes.CDLL:
             "orc_svm_eval": ([_i64, _i64, _I32P, _I32P, _F64P, _F64P, _f64, _F64P, _F64P], None),
             "orc_cd_async_omp": ([ctypes.c_int, _i64, _i64, _I32P, _I32P, _F64P, _F64P, _f64, _u64,
                                   _u64, _u64, ctypes.c_int, _F64P, _F64P], ctypes.c_int),
+            "orc_async_coords": ([_u64, ctypes.c_uint32, _u64, _u64, ctypes.c_uint32, _I32P], None),
+            "orc_cd_list": ([ctypes.c_int, _i64, _i64, _I32P, _I32P, _F64P, _F64P, _f64, _I32P, _u64,
+                             _F64P, _F64P], None),
         }
         for name, (a, r) in sig.items():
             f = getattr(L, name)
@@ -97,6 +100,12 @@ def soft_threshold(t, v):
     return lib().orc_soft_threshold(t, v)
 
 
+def async_coords(seed, worker, k0, count, n) -> np.ndarray:
+    out = np.empty(count, np.int32)
+    lib().orc_async_coords(seed, worker, k0, count, n, _p(out, ctypes.c_int32))
+    return out
+
+
 def omega(nidx, idx) -> int:
     idx = i32(idx)
     return lib().orc_omega(nidx, idx.size, _p(idx, ctypes.c_int32))
@@ -142,11 +151,18 @@ class CD:
                          self.lam, seed, tau, step0, nsteps, _p(self.x, ctypes.c_double),
                          _p(self.r, ctypes.c_double))
 
-    def run_async(self, seed, update0, n_updates, threads):
+    def run_async(self, seed, k0, per_worker, threads):
+        """threads workers, updates [k0, k0+per_worker) of each worker's stream"""
         return lib().orc_cd_async_omp(self.loss, self.m, self.n, *self._args(),
-                                      _p(self.L, ctypes.c_double), self.lam, seed, update0,
-                                      n_updates, threads, _p(self.x, ctypes.c_double),
+                                      _p(self.L, ctypes.c_double), self.lam, seed, k0,
+                                      per_worker, threads, _p(self.x, ctypes.c_double),
                                       _p(self.r, ctypes.c_double))
+
+    def run_list(self, coords):
+        c = i32(coords)
+        lib().orc_cd_list(self.loss, self.m, self.n, *self._args(), _p(self.L, ctypes.c_double),
+                          self.lam, _p(c, ctypes.c_int32), c.size, _p(self.x, ctypes.c_double),
+                          _p(self.r, ctypes.c_double))
 
     def eval(self, x=None) -> dict:
         x = self.x if x is None else f64(x)

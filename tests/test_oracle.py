@@ -251,6 +251,43 @@ def test_svm_w_identity(orc):
     assert np.allclose(s.w, (a * y) @ X / (0.05 * ns), rtol=1e-14, atol=1e-16)
 
 
+def test_async_coords_multiply_shift_exact(orc):
+    """The 64x32-bit multiply-shift map against exact integer arithmetic on
+    the Philox words, and distinct workers get distinct streams."""
+    n = 1_000_003
+    seed = 0x1234_5678_9ABC
+    for w in (0, 1, 77):
+        c = orc.async_coords(seed, w, 5, 40, n)
+        for k in range(5, 45):
+            blk = k >> 1
+            o = orc.philox([blk & 0xFFFFFFFF, blk >> 32, w, 0x50434441],
+                           [seed & 0xFFFFFFFF, seed >> 32])
+            lo, hi = (o[2], o[3]) if k & 1 else (o[0], o[1])
+            assert c[k - 5] == ((hi << 32 | lo) * n) >> 64
+    a, b = orc.async_coords(seed, 0, 0, 64, n), orc.async_coords(seed, 1, 0, 64, n)
+    assert (a != b).any()
+
+
+def test_async_coords_uniform(orc):
+    from scipy.stats import chisquare
+    n = 50
+    c = np.concatenate([orc.async_coords(3, w, 0, 4000, n) for w in range(5)])
+    assert c.min() >= 0 and c.max() < n
+    assert chisquare(np.bincount(c, minlength=n)).pvalue > 1e-3
+
+
+def test_cd_list_equals_serial_run(orc):
+    rng = np.random.default_rng(12)
+    m, n = 70, 90
+    ptr, rows, vals = rand_csc(rng, m, n, 7)
+    b = rng.standard_normal(m)
+    a = orc.CD(0, m, n, ptr, rows, vals, b, 0.1)
+    s = orc.CD(0, m, n, ptr, rows, vals, b, 0.1)
+    a.run_list(orc.sample_steps(4, 0, 3000, n, 1))
+    s.run(seed=4, tau=1, step0=0, nsteps=3000)
+    assert (a.x == s.x).all() and (a.r == s.r).all()
+
+
 def test_async_one_thread_equals_serial(orc):
     rng = np.random.default_rng(9)
     m, n = 70, 90
@@ -258,8 +295,8 @@ def test_async_one_thread_equals_serial(orc):
     b = rng.standard_normal(m)
     a = orc.CD(0, m, n, ptr, rows, vals, b, 0.1)
     s = orc.CD(0, m, n, ptr, rows, vals, b, 0.1)
-    assert a.run_async(seed=4, update0=0, n_updates=5000, threads=1) == 1
-    s.run(seed=4, tau=1, step0=0, nsteps=5000)
+    assert a.run_async(seed=4, k0=100, per_worker=5000, threads=1) == 1
+    s.run_list(orc.async_coords(4, 0, 100, 5000, n))
     assert (a.x == s.x).all() and (a.r == s.r).all()
 
 
@@ -269,5 +306,5 @@ def test_async_threads_converge(orc):
     ptr, rows, vals, b, x_star, F_star = planted_lasso(rng, m, n, 8, 15, lam)
     a = orc.CD(0, m, n, ptr, rows, vals, b, lam, beta=orc.eso_beta(
         orc.omega(m, rows), 4, n))
-    a.run_async(seed=4, update0=0, n_updates=400 * n, threads=4)
+    a.run_async(seed=4, k0=0, per_worker=100 * n, threads=4)
     assert a.eval()["primal"] == pytest.approx(F_star, rel=1e-8)
