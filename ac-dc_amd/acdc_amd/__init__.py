@@ -67,7 +67,8 @@ class _Report(ctypes.Structure):
                 ("primal", "dual", "gap", "rel_gap", "grad_inf", "x_l1", "res_sq",
                  "kkt", "accuracy", "nnz_x")] + \
                [("updates", ctypes.c_uint64), ("epochs", ctypes.c_double),
-                ("kernel_ms", ctypes.c_double), ("eval_ms", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double), ("eval_ms", ctypes.c_double),
+                ("step_ms", ctypes.c_double), ("step_launches", ctypes.c_uint64)]
 
 
 class GenLassoCfg(ctypes.Structure):

@@ -203,7 +203,11 @@ struct acdc_solver {
   uint64_t step = 0;
   uint64_t updates = 0;
   uint64_t updates_since_refresh = 0;
-  double kernel_ms = 0, eval_ms = 0;
+  double kernel_ms = 0, eval_ms = 0, step_ms = 0;
+  uint64_t step_launches = 0;
+  // event pairs around the update-kernel launches of one acdc_run_updates
+  std::vector<hipEvent_t> step_ev;
+  size_t step_ev_used = 0;
   std::vector<void *> allocs;
 };
 
@@ -392,6 +396,18 @@ acdc_status timed_begin(acdc_solver *s) {
   HIP_TRY(hipEventRecord(s->ev0, s->stream));
   return ACDC_OK;
 }
+// records the next event of the solver's pool on its stream; pairs of marks
+// bracket the update-kernel launches of acdc_run_updates
+acdc_status step_mark(acdc_solver *s) {
+  if (s->step_ev_used == s->step_ev.size()) {
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreate(&e));
+    s->step_ev.push_back(e);
+  }
+  HIP_TRY(hipEventRecord(s->step_ev[s->step_ev_used++], s->stream));
+  return ACDC_OK;
+}
+
 acdc_status timed_end(acdc_solver *s, double *acc) {
   HIP_TRY(hipEventRecord(s->ev1, s->stream));
   HIP_TRY(hipEventSynchronize(s->ev1));
@@ -405,6 +421,7 @@ void destroy_solver(acdc_solver *s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void *p : s->allocs) (void)hipFree(p);
+  for (hipEvent_t e : s->step_ev) (void)hipEventDestroy(e);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -591,16 +608,25 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
       const uint64_t left = period > s->updates_since_refresh ? period - s->updates_since_refresh : 1;
       chunk = std::min(chunk, (left + tau - 1) / tau);
     }
+    if ((st = step_mark(s)) != ACDC_OK) return st;
     if (s->async) {
-      for (uint64_t k = 0; k < chunk; k += kAsyncPerLaunch)
-        dispatch3<LaunchAsync>(s, s->lanes, s->step + k, std::min(kAsyncPerLaunch, chunk - k));
+      // equal launches of at most kAsyncPerLaunch updates per worker
+      const uint64_t nl = (chunk + kAsyncPerLaunch - 1) / kAsyncPerLaunch;
+      const uint64_t per = (chunk + nl - 1) / nl;
+      for (uint64_t k = 0; k < chunk; k += per)
+        dispatch3<LaunchAsync>(s, s->lanes, s->step + k, std::min(per, chunk - k));
+      s->step_launches += nl;
     } else if (tau == 1) {
       dispatch3<LaunchSerial>(s, s->lanes, s->step, chunk);
+      s->step_launches += 1;
     } else if (s->persistent) {
       if ((st = run_persistent(s, s->step, chunk)) != ACDC_OK) return st;
+      s->step_launches += s->persistent ? (chunk + kStepsPerLaunch - 1) / kStepsPerLaunch : 2 * chunk;
     } else {
       for (uint64_t k = 0; k < chunk; ++k) dispatch3<LaunchStep>(s, s->lanes, s->step + k, 3);
+      s->step_launches += 2 * chunk;
     }
+    if ((st = step_mark(s)) != ACDC_OK) return st;
     s->step += chunk;
     s->updates += chunk * tau;
     s->updates_since_refresh += chunk * tau;
@@ -610,6 +636,12 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
   }
   HIP_TRY(hipGetLastError());
   if ((st = timed_end(s, &s->kernel_ms)) != ACDC_OK) return st;
+  for (size_t e = 0; e + 1 < s->step_ev_used; e += 2) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, s->step_ev[e], s->step_ev[e + 1]));
+    s->step_ms += ms;
+  }
+  s->step_ev_used = 0;
   if (s->persistent) {
     GridSync g{};
     HIP_TRY(hipMemcpy(&g, s->gsync, sizeof g, hipMemcpyDeviceToHost));
@@ -679,6 +711,8 @@ extern "C" acdc_status acdc_report_get(acdc_solver *s, acdc_report *out) {
   r.epochs = (double)s->updates / (double)s->ncoord;
   r.kernel_ms = s->kernel_ms;
   r.eval_ms = s->eval_ms;
+  r.step_ms = s->step_ms;
+  r.step_launches = s->step_launches;
   *out = r;
   return ACDC_OK;
 }

@@ -2,17 +2,20 @@
 """PCDM hot-path benchmark on one MI355X.
 
 Workload: the 200,000 x 1,000,000 Lasso (column nnz truncated Zipf(1.1) on
-[5,200], unit columns, 5,000-sparse planted x*, lambda = 0.05 ||A^T b||_inf),
-tau = 4096 coordinates per synchronous step with the ESO beta.  One "step"
-of this benchmark is one epoch: n = 1,000,000 coordinate updates (245
-synchronous tau-steps) over data resident in HBM.
+[5,200], unit columns, 5,000-sparse planted x*, lambda = 0.05 ||A^T b||_inf)
+with tau = 4096 and the ESO beta of tau.  One "step" of this benchmark is one
+epoch: n = 1,000,000 coordinate updates over data resident in HBM.
 
-Prints one JSON line: coordinate updates/s of the FP32 + FP64-refresh path
-(value; persistent cooperative kernel), the same for the plain FP64 path
-(one gather + one scatter launch per step), the time each needs from x = 0
-to an FP64 relative duality gap <= 1e-6 (gap checked every 5 epochs), the
-roofline of the dominant kernel, and the CPU baseline (the oracle's OpenMP
-asynchronous restatement of the reference solver) on the host cores.
+value: coordinate updates/s of the FP32 + FP64-refresh path (4096
+asynchronous workers, FP32 residual corrections anchored on an FP64 refresh
+every 5 epochs), from x = 0.  Beside it: the plain FP64 path (synchronous
+4096-coordinate steps, one gather + one scatter launch per step), the
+synchronous FP32 persistent kernel, the wall time each of the FP32 and plain
+FP64 paths needs from x = 0 to an FP64 relative duality gap <= 1e-6 (gap
+evaluated every 5 epochs, evaluations included), the roofline of the
+dominant kernel (k_async, timed with HIP events on the solver's stream), and
+the CPU baseline (the oracle's OpenMP asynchronous restatement of the
+reference solver) on the host cores.
 """
 import argparse
 import json
@@ -32,38 +35,45 @@ def log(msg):
 
 
 def time_to_gap(solver, n, target, check_every, max_epochs):
-    """Wall time from x = 0 to rel gap <= target, FP64 gap every check_every epochs."""
+    """Wall time from x = 0 to rel gap <= target: one run_updates per epoch,
+    the FP64 gap every check_every epochs (evaluations included)."""
     import numpy as np
     solver.x = np.zeros(n)
     solver.synchronize()
     t0 = time.perf_counter()
     epochs, rep = 0, solver.report()
     while epochs < max_epochs and rep["rel_gap"] > target:
-        solver.run_updates(check_every * n)
+        for _ in range(check_every):
+            solver.run_updates(n)
         epochs += check_every
         rep = solver.report()
     return time.perf_counter() - t0, epochs, rep
 
 
 def throughput(solver, n, epochs):
-    """Coordinate updates/s over `epochs` epochs (wall clock around them)."""
+    """Coordinate updates/s over `epochs` epochs (wall clock around them), and
+    the device time and launch count of the update kernels inside them."""
     solver.synchronize()
-    u0 = solver.report()["updates"]
+    r0 = solver.report()
     t0 = time.perf_counter()
     for _ in range(epochs):
         solver.run_updates(n)
     solver.synchronize()
     wall = time.perf_counter() - t0
-    upd = solver.report()["updates"] - u0
-    return upd / wall, wall, upd
+    r1 = solver.report()
+    upd = r1["updates"] - r0["updates"]
+    return {"value": upd / wall, "wall": wall, "updates": upd,
+            "step_ms": r1["step_ms"] - r0["step_ms"],
+            "launches": r1["step_launches"] - r0["step_launches"]}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50, help="timed epochs")
+    ap.add_argument("--steps", type=int, default=100, help="timed epochs")
     ap.add_argument("--warmup", type=int, default=2, help="untimed epochs")
     ap.add_argument("--tau", type=int, default=4096)
+    ap.add_argument("--lanes", type=int, default=32, help="lanes per coordinate (0 = auto)")
     ap.add_argument("--m", type=int, default=200_000)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--gap-target", type=float, default=1e-6)
@@ -90,35 +100,30 @@ def main():
     n, tau = d.n, args.tau
     mean_nnz = d.nnz / d.n
 
-    def make(precision, launch, refresh=5):
+    def make(precision, mode, launch=acdc.LAUNCH_AUTO, refresh=5, lanes=0):
         return acdc.Solver(prob, acdc.LOSS_L1_SQUARE, lam=d.lam, tau=tau, seed=2,
-                           precision=precision, refresh_period=refresh, launch_mode=launch)
+                           precision=precision, refresh_period=refresh, step_mode=mode,
+                           launch_mode=launch, lanes_per_coord=lanes)
 
-    # ---- FP32 + FP64 refresh, persistent grid: throughput over the timed epochs
-    s32 = make(acdc.PREC_FP32_REFRESH, acdc.LAUNCH_AUTO)
+    # ---- FP32 + FP64 refresh, asynchronous workers: the timed epochs
+    s32 = make(acdc.PREC_FP32_REFRESH, acdc.STEP_ASYNC, lanes=args.lanes)
     beta = s32.beta
     s32.run_updates(args.warmup * n)
     s32.x = np.zeros(n)
-    value, wall, upd = throughput(s32, n, args.steps)
-    log(f"[bench] fp32 persistent: {upd} updates in {wall:.3f}s = {value:.4e} upd/s")
+    tp = throughput(s32, n, args.steps)
+    value, wall = tp["value"], tp["wall"]
+    log(f"[bench] fp32 async: {tp['updates']} updates in {wall:.3f}s = {value:.4e} upd/s")
     s32.close()
 
-    # ---- dominant kernel: one persistent launch per epoch (refresh off),
-    #      timed with HIP events on the solver's stream
-    sk = make(acdc.PREC_FP32_REFRESH, acdc.LAUNCH_AUTO, refresh=0)
-    sk.run_updates(n)
-    k0 = sk.report()["kernel_ms"]
-    reps = 8
-    for _ in range(reps):
-        sk.run_updates(n)
-    launch_ms = (sk.report()["kernel_ms"] - k0) / reps
-    steps_per_launch = -(-n // tau)
-    bytes_per_update = 8.0 * mean_nnz + 32.0  # idx+val per nnz, col ptrs, L_j, x_j r/w
-    bytes_per_launch = steps_per_launch * tau * bytes_per_update
+    # ---- dominant kernel: k_async over the same timed epochs (one launch per
+    #      epoch), HIP events on the solver's stream around the launches
+    launch_ms = tp["step_ms"] / max(1, tp["launches"])
+    upd_per_launch = tp["updates"] / max(1, tp["launches"])
+    bytes_per_update = 8.0 * mean_nnz + 32.0  # (row, value) per nnz; ptr pair, L_j, gref_j, x_j
+    bytes_per_launch = upd_per_launch * bytes_per_update
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    log(f"[bench] k_sync_persistent: {launch_ms:.3f} ms per launch ({steps_per_launch} steps), "
+    log(f"[bench] k_async: {launch_ms:.3f} ms per launch ({upd_per_launch:.0f} updates), "
         f"{achieved:.1f} GB/s")
-    sk.close()
 
     result = {
         "metric": METRIC, "value": value, "unit": "coord updates/s", "n_gpus": 1,
@@ -127,28 +132,39 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"lasso {d.m}x{d.n} zipf1.1[5,200] tau{tau}", "m": d.m, "n": d.n,
                    "nnz": d.nnz, "omega": prob.omega, "tau": tau, "beta": beta,
-                   "lambda": d.lam,
+                   "lambda": d.lam, "mode": "asynchronous workers",
                    "precision": "fp32 dots/scatter on an fp64-anchored residual, fp64 x, "
                                 "fp64 refresh every 5 epochs",
-                   "step": "one epoch = n coordinate updates"},
+                   "step": "one epoch = n coordinate updates, from x = 0"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sync_persistent<float,64,0,2>", "launch_ms": launch_ms,
-                     "steps_per_launch": steps_per_launch,
+                     "kernel": "k_async<float,G,0,4,true>", "launch_ms": launch_ms,
+                     "updates_per_launch": upd_per_launch,
                      "bytes_per_update": bytes_per_update, "bytes_per_launch": bytes_per_launch},
     }
 
-    # ---- plain FP64 path: per-step launches
-    s64 = make(acdc.PREC_FP64, acdc.LAUNCH_PER_STEP)
+    # ---- plain FP64 path: synchronous steps, per-step launches
+    s64 = make(acdc.PREC_FP64, acdc.STEP_SYNC, launch=acdc.LAUNCH_PER_STEP)
     s64.run_updates(args.warmup * n)
     s64.x = np.zeros(n)
-    v64, wall64, _ = throughput(s64, n, max(1, args.steps // 5))
-    result["fp64_plain"] = {"value": v64, "unit": "coord updates/s", "launch": "per step"}
-    log(f"[bench] fp64 plain: {v64:.4e} upd/s")
+    tp64 = throughput(s64, n, max(1, args.steps // 5))
+    result["fp64_plain"] = {"value": tp64["value"], "unit": "coord updates/s",
+                            "mode": "synchronous tau-steps, gather + scatter launch per step"}
+    log(f"[bench] fp64 plain: {tp64['value']:.4e} upd/s")
+
+    # ---- synchronous FP32 steps in the persistent cooperative kernel
+    ss = make(acdc.PREC_FP32_REFRESH, acdc.STEP_SYNC)
+    ss.run_updates(args.warmup * n)
+    ss.x = np.zeros(n)
+    tps = throughput(ss, n, max(1, args.steps // 5))
+    result["fp32_sync_persistent"] = {"value": tps["value"], "unit": "coord updates/s",
+                                      "launches_per_epoch": tps["launches"] / max(1, args.steps // 5)}
+    log(f"[bench] fp32 sync: {tps['value']:.4e} upd/s")
+    ss.close()
 
     x32 = None
     if not args.no_ttg:
-        s = make(acdc.PREC_FP32_REFRESH, acdc.LAUNCH_AUTO)
+        s = make(acdc.PREC_FP32_REFRESH, acdc.STEP_ASYNC, lanes=args.lanes)
         tt, ep, rep = time_to_gap(s, n, args.gap_target, args.gap_every, args.max_epochs)
         result["time_to_gap"] = {"seconds": tt, "epochs": ep, "rel_gap": rep["rel_gap"],
                                  "reached": rep["rel_gap"] <= args.gap_target,

@@ -110,8 +110,12 @@ typedef struct acdc_report {
   double nnz_x;     /* number of nonzero coordinates */
   uint64_t updates; /* coordinate updates applied since create */
   double epochs;    /* updates / n */
-  double kernel_ms; /* device time of the update kernels so far */
-  double eval_ms;   /* device time of refresh + evaluation kernels so far */
+  double kernel_ms; /* device time inside acdc_run_updates so far (update
+                       kernels and the refreshes they trigger) */
+  double eval_ms;   /* device time of explicit refreshes and reports so far */
+  double step_ms;   /* device time of the update kernels alone so far */
+  uint64_t step_launches; /* update-kernel launches so far (two per step with
+                             per-step launches) */
 } acdc_report;
 
 typedef struct acdc_problem acdc_problem;
