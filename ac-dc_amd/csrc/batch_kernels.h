@@ -39,20 +39,6 @@ struct alignas(32) CoordRec {
   int32_t len;  // entries in the column
 };
 
-template <typename T>
-struct Entry;  // one (row, value) pair of a column
-template <>
-struct alignas(8) Entry<float> {
-  int32_t row;
-  float val;
-};
-template <>
-struct alignas(16) Entry<double> {
-  int32_t row;
-  int32_t pad;
-  double val;
-};
-
 struct BatchInstance {
   CoordRec *cs;          // n records
   const void *ent;       // Entry<T>[nnz]

@@ -31,6 +31,23 @@ struct Seg {
 
 constexpr int kBlock = 256;  // threads per block
 
+// One (index, value) pair of a segment, interleaved so that a lane's entry
+// is one 8-byte (FP32) or 16-byte (FP64) load and a segment's entries share
+// cache lines.
+template <typename T>
+struct Entry;
+template <>
+struct alignas(8) Entry<float> {
+  int32_t row;
+  float val;
+};
+template <>
+struct alignas(16) Entry<double> {
+  int32_t row;
+  int32_t pad;
+  double val;
+};
+
 template <typename T>
 struct StepArgs {
   Seg<T> A;
@@ -303,105 +320,179 @@ __global__ __launch_bounds__(64) void k_serial(StepArgs<T> a, uint64_t step0, ui
 // level, and written with hardware atomic adds (x_j += d, r += c a_j), so
 // no update is lost.  At most tau updates are in flight at once.
 //
-// Pipelining: everything an update needs apart from the residual and x_j --
-// its coordinate, segment bounds, L_j, the anchored gradient, the label and
-// the first R*G segment entries -- is loaded one update ahead (the entries)
-// or two ahead (the rest), into two alternating register sets.  Those
-// prefetches are issued just before the update's own gather, so a single
-// wait per update covers every load in flight (vector memory counters drain
-// in order, so waiting for a gather issued before the prefetches would wait
-// for them anyway) and the critical path of an update is one round trip.
-// With TAIL = false every segment fits the R*G cached entries (the host
-// checks the longest), which keeps variable-length loops out of the update.
-// With tau = 1 the worker also waits for its scatter before the next gather,
-// which makes the run the serial method exactly.
-constexpr int kAsyncBlock = 256;
+// Layout for random coordinates: all per-coordinate state is one 32-byte
+// record (x_j, the anchored gradient or L_j, the label, the segment's offset
+// and length) and the segment is a run of interleaved (index, value) pairs,
+// so an update touches one record sector and the few lines of its segment
+// instead of five scattered arrays.  FP32 records keep L_j in FP32 (the
+// fixed point of the proximal step does not depend on the step length);
+// FP64 records keep it in FP64, so that one worker reproduces the serial
+// FP64 method.
+//
+// Pipelining: the record and first R*G entries of update k+1 are loaded
+// during update k (the record one update earlier still), just before update
+// k's own gather, so a single wait per update covers every load in flight
+// (vector memory counters drain in order) and the critical path of an
+// update is one round trip.  With TAIL = false every segment fits the R*G
+// cached entries (the host checks the longest), which keeps variable-length
+// loops out of the update.  With tau = 1 the worker also waits for its
+// scatter before the next gather, which makes the run the serial method.
+template <typename T>
+struct AsyncRec;
+template <>
+struct alignas(32) AsyncRec<float> {
+  double x;
+  double gref;  // A^T r64 at the last refresh
+  float L;
+  float y;      // SVM label
+  int32_t p0, len;
+};
+template <>
+struct alignas(32) AsyncRec<double> {
+  double x;
+  double L;
+  float y;
+  float pad;
+  int32_t p0, len;
+};
+__device__ __forceinline__ double rec_L(const AsyncRec<float> &r) { return (double)r.L; }
+__device__ __forceinline__ double rec_L(const AsyncRec<double> &r) { return r.L; }
+__device__ __forceinline__ double rec_gref(const AsyncRec<float> &r) { return r.gref; }
+__device__ __forceinline__ double rec_gref(const AsyncRec<double> &) { return 0.0; }
 
-template <typename T, int R>
-struct AsyncCoord {
-  uint32_t j;
-  int32_t p0, p1;
-  double L, gref, y;
-  int32_t ci[R];
-  T cv[R];
+template <typename T>
+struct AsyncArgs {
+  AsyncRec<T> *rec;
+  const Entry<T> *ent;
+  T *res;  // FP64: residual; FP32: correction dr
+  StepScalars sc;
+  uint64_t seed;
+  int64_t tau;
+  uint32_t n;
 };
 
+constexpr int kAsyncBlock = 256;
+
 template <typename T, int G, int LOSS, int R, bool TAIL>
-__global__ __launch_bounds__(kAsyncBlock) void k_async(StepArgs<T> a, uint64_t k0,
+__global__ __launch_bounds__(kAsyncBlock) void k_async(AsyncArgs<T> a, uint64_t k0,
                                                        uint64_t count) {
   const int lane = threadIdx.x & (G - 1);
   const int64_t w = ((int64_t)blockIdx.x * kAsyncBlock + threadIdx.x) / G;
   if (w >= a.tau || count == 0) return;  // whole groups leave together
-  const uint32_t n = a.perm.n;
   const bool serial = a.tau == 1;
-  // stage 1: coordinate and per-coordinate scalars of update k
-  auto bounds = [&](uint64_t k, AsyncCoord<T, R> &c) {
-    c.j = async_coord(a.seed, (uint32_t)w, k0 + k, n);
-    c.p0 = a.A.ptr[c.j];
-    c.p1 = a.A.ptr[c.j + 1];
-    c.L = a.L[c.j];
-    c.gref = a.gref ? a.gref[c.j] : 0.0;
-    c.y = LOSS == kSvm ? a.y[c.j] : 0.0;
+  struct Slot {
+    uint32_t j;
+    AsyncRec<T> r;
+    int32_t ci[R];
+    T cv[R];
   };
-  // stage 2: the cached segment entries
-  auto entries = [&](AsyncCoord<T, R> &c) {
+  auto record = [&](uint64_t k, Slot &c) {
+    c.j = async_coord(a.seed, (uint32_t)w, k0 + k, a.n);
+    c.r = a.rec[c.j];
+  };
+  auto entries = [&](Slot &c) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int32_t p = c.p0 + lane + r * G;
-      c.ci[r] = p < c.p1 ? a.A.idx[p] : 0;
-      c.cv[r] = p < c.p1 ? a.A.val[p] : (T)0;
+      const int32_t q = lane + r * G;
+      if (q < c.r.len) {
+        const Entry<T> e = a.ent[c.r.p0 + q];
+        c.ci[r] = e.row;
+        c.cv[r] = e.val;
+      } else {
+        c.ci[r] = 0;
+        c.cv[r] = (T)0;
+      }
     }
   };
-  // update k on cur; meanwhile nxt (update k+1) gets its entries and the
-  // scalars of update k+2 land in ahead, which becomes cur's next content
-  auto update = [&](uint64_t k, AsyncCoord<T, R> &cur, AsyncCoord<T, R> &nxt,
-                    AsyncCoord<T, R> &ahead) {
-    // prefetches first, then the gather: every load of the update is in
-    // flight together and one wait covers them (and the previous scatter)
+  // update k on cur while nxt (update k+1) gets its entries and the record
+  // of update k+2 lands in ahead, which then becomes cur's next content
+  auto update = [&](uint64_t k, Slot &cur, Slot &nxt, Slot &ahead) {
     if (k + 1 < count) entries(nxt);
-    if (k + 2 < count) bounds(k + 2, ahead);
+    if (k + 2 < count) record(k + 2, ahead);
     T v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)  // padding lanes issue no load
-      v[r] = cur.p0 + lane + r * G < cur.p1 ? ld_coherent(&a.res[cur.ci[r]]) : (T)0;
-    const double xj = ld_coherent(&a.x[cur.j]);
+      v[r] = lane + r * G < cur.r.len ? ld_coherent(&a.res[cur.ci[r]]) : (T)0;
+    const double xj = ld_coherent(&a.rec[cur.j].x);
     T acc = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) acc += cur.cv[r] * v[r];
     if (TAIL)
-      for (int32_t p = cur.p0 + lane + R * G; p < cur.p1; p += G)
-        acc += a.A.val[p] * ld_coherent(&a.res[a.A.idx[p]]);
+      for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
+        const Entry<T> e = a.ent[cur.r.p0 + q];
+        acc += e.val * ld_coherent(&a.res[e.row]);
+      }
     acc = group_sum<G>(acc);
     double c;
-    const double g = a.gref ? cur.gref + (double)acc : (double)acc;
-    const double d = prox_delta<LOSS>(g, xj, cur.L, cur.y, a.sc, &c);
-    if (lane == 0 && d != 0.0) unsafeAtomicAdd(&a.x[cur.j], d);
+    const double g = rec_gref(cur.r) + (double)acc;
+    const double d = prox_delta<LOSS>(g, xj, rec_L(cur.r), (double)cur.r.y, a.sc, &c);
+    if (lane == 0 && d != 0.0) unsafeAtomicAdd(&a.rec[cur.j].x, d);
     if (c != 0.0) {
       const T ct = (T)c;
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        if (cur.p0 + lane + r * G < cur.p1) unsafeAtomicAdd(&a.res[cur.ci[r]], ct * cur.cv[r]);
+        if (lane + r * G < cur.r.len) unsafeAtomicAdd(&a.res[cur.ci[r]], ct * cur.cv[r]);
       if (TAIL)
-        for (int32_t p = cur.p0 + lane + R * G; p < cur.p1; p += G)
-          unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
+        for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
+          const Entry<T> e = a.ent[cur.r.p0 + q];
+          unsafeAtomicAdd(&a.res[e.row], ct * e.val);
+        }
     }
     if (serial) wait_memory();
-    // the scalars of update k+2 move into cur, whose update is finished
     cur.j = ahead.j;
-    cur.p0 = ahead.p0;
-    cur.p1 = ahead.p1;
-    cur.L = ahead.L;
-    cur.gref = ahead.gref;
-    cur.y = ahead.y;
+    cur.r = ahead.r;
   };
-  AsyncCoord<T, R> s0, s1, ahead;
-  bounds(0, s0);
+  Slot s0, s1, ahead;
+  record(0, s0);
   entries(s0);
-  if (count > 1) bounds(1, s1);
+  if (count > 1) record(1, s1);
   for (uint64_t k = 0; k < count; k += 2) {
     update(k, s0, s1, ahead);
     if (k + 1 < count) update(k + 1, s1, s0, ahead);
   }
+}
+
+// record set-up and the x / anchor exchange with the solver's plain arrays
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rec_init(AsyncRec<T> *__restrict__ rec, int64_t n,
+                                                     const int32_t *__restrict__ ptr,
+                                                     const double *__restrict__ L,
+                                                     const double *__restrict__ y) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  AsyncRec<T> r{};
+  r.L = L[j];
+  r.y = y ? (float)y[j] : 0.0f;
+  r.p0 = ptr[j];
+  r.len = ptr[j + 1] - ptr[j];
+  rec[j] = r;
+}
+// x (and the FP32 anchor gref) into the records
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rec_put(AsyncRec<T> *__restrict__ rec, int64_t n,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ gref) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  rec[j].x = x[j];
+  if constexpr (sizeof(T) == 4) rec[j].gref = gref[j];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rec_get_x(const AsyncRec<T> *__restrict__ rec,
+                                                      int64_t n, double *__restrict__ x) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j < n) x[j] = rec[j].x;
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pack_entries(const int32_t *__restrict__ idx,
+                                                         const T *__restrict__ val, int64_t nnz,
+                                                         Entry<T> *__restrict__ ent) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nnz) return;
+  Entry<T> e{};
+  e.row = idx[p];
+  e.val = val[p];
+  ent[p] = e;
 }
 
 // ------------------------------------------------------------------- setup
@@ -503,33 +594,58 @@ __global__ __launch_bounds__(kBlock) void k_eval_coord(Seg<T> A, int64_t nseg,
                                                        const double *__restrict__ y,
                                                        double *__restrict__ gref,
                                                        double *__restrict__ partial) {
+  // Grid-stride over segments, software-pipelined: the bounds, first
+  // R*G entries and x_c of the next segment load while the
+  // current one gathers r64, so a segment costs about one round trip.
+  constexpr int R = 4;
   double v[kStats];
   for (int s = 0; s < kStats; ++s) v[s] = 0;
   if (LOSS == kNnls) v[2] = INFINITY;
   const int lane = threadIdx.x & (G - 1);
   const int64_t ngroups = (int64_t)gridDim.x * (kBlock / G);
-  for (int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G; c < nseg; c += ngroups) {
+  int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G;
+  int32_t p0 = 0, p1 = 0;
+  int32_t ci[R];
+  T cv[R];
+  double xc = 0, yc = 0;
+  auto load = [&](int64_t col) {
+    p0 = A.ptr[col];
+    p1 = A.ptr[col + 1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t p = p0 + lane + r * G;
+      ci[r] = p < p1 ? A.idx[p] : 0;
+      cv[r] = p < p1 ? A.val[p] : (T)0;
+    }
+    xc = x[col];
+    yc = LOSS == kSvm ? y[col] : 0.0;
+  };
+  if (c < nseg) load(c);
+  for (; c < nseg; c += ngroups) {
     double g = 0;
-    for (int32_t p = A.ptr[c] + lane; p < A.ptr[c + 1]; p += G)
-      g += (double)A.val[p] * r64[A.idx[p]];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (p0 + lane + r * G < p1) g += (double)cv[r] * r64[ci[r]];
+    for (int32_t p = p0 + lane + R * G; p < p1; p += G) g += (double)A.val[p] * r64[A.idx[p]];
+    const double xcur = xc, ycur = yc;
+    if (c + ngroups < nseg) load(c + ngroups);
     g = group_sum<G>(g);
     if (lane == 0) {
       if (gref) gref[c] = g;
-      const double xc = x[c];
       if constexpr (LOSS == kLasso) {
         v[0] = fmax(v[0], fabs(g));
-        v[1] += fabs(xc);
-        v[2] += xc != 0.0;
+        v[1] += fabs(xcur);
+        v[2] += xcur != 0.0;
       } else if constexpr (LOSS == kNnls) {
-        v[0] = fmax(v[0], fabs(fmin(xc, g)));
-        v[1] += xc * g;
+        v[0] = fmax(v[0], fabs(fmin(xcur, g)));
+        v[1] += xcur * g;
         v[2] = fmin(v[2], g);
         v[3] = fmax(v[3], fabs(g));
-        v[4] += fabs(xc);
-        v[5] += xc != 0.0;
+        v[4] += fabs(xcur);
+        v[5] += xcur != 0.0;
       } else {
-        const double al = xc > 1.0 ? 1.0 : (xc < 0.0 ? 0.0 : xc);
-        const double ym = y[c] * g;
+        const double al = xcur > 1.0 ? 1.0 : (xcur < 0.0 ? 0.0 : xcur);
+        const double ym = ycur * g;
         const double h = 1.0 - ym;
         v[0] += h > 0 ? h : 0.0;
         v[1] += ym > 0;

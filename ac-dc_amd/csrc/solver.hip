@@ -186,6 +186,12 @@ struct acdc_solver {
   bool fp32 = false;
   bool persistent = false;
   bool async = false;  // ACDC_STEP_ASYNC
+  // asynchronous mode: per-coordinate records and interleaved entries; the
+  // records hold x while workers run, the plain x array between runs
+  void *rec = nullptr, *ent = nullptr;
+  bool rec_cur = false;  // records hold the current x (and anchor)
+  bool x_cur = true;     // the plain x array is current
+  bool stats_cur = false;  // FP32: stats of the last refresh describe the current x
   int64_t maxlen = 0;  // longest segment
   int64_t coresident_blocks = 0;
   hipStream_t stream = nullptr;
@@ -202,7 +208,7 @@ struct acdc_solver {
   GridSync *gsync = nullptr;
   uint64_t step = 0;
   uint64_t updates = 0;
-  uint64_t updates_since_refresh = 0;
+  uint64_t steps_since_refresh = 0;  // steps (async: updates per worker)
   double kernel_ms = 0, eval_ms = 0, step_ms = 0;
   uint64_t step_launches = 0;
   // event pairs around the update-kernel launches of one acdc_run_updates
@@ -306,15 +312,49 @@ struct LaunchStep {
 template <typename T, int G, int LOSS>
 struct LaunchAsync {
   static void run(const acdc_solver *s, uint64_t k0, uint64_t count) {
+    AsyncArgs<T> a;
+    a.rec = static_cast<AsyncRec<T> *>(s->rec);
+    a.ent = static_cast<const Entry<T> *>(s->ent);
+    a.res = static_cast<T *>(s->res);
+    a.sc = scalars(s);
+    a.seed = s->set.seed;
+    a.tau = s->set.tau;
+    a.n = (uint32_t)s->ncoord;
     const unsigned blocks = (unsigned)div_up(s->set.tau * G, kAsyncBlock);
     if (s->maxlen <= kAsyncChunks * G)
-      k_async<T, G, LOSS, kAsyncChunks, false>
-          <<<blocks, kAsyncBlock, 0, s->stream>>>(step_args<T>(s), k0, count);
+      k_async<T, G, LOSS, kAsyncChunks, false><<<blocks, kAsyncBlock, 0, s->stream>>>(a, k0, count);
     else
-      k_async<T, G, LOSS, kAsyncChunks, true>
-          <<<blocks, kAsyncBlock, 0, s->stream>>>(step_args<T>(s), k0, count);
+      k_async<T, G, LOSS, kAsyncChunks, true><<<blocks, kAsyncBlock, 0, s->stream>>>(a, k0, count);
   }
 };
+
+// x / anchor exchange between the plain arrays and the async records
+acdc_status ensure_rec(acdc_solver *s) {
+  if (!s->async || s->rec_cur) return ACDC_OK;
+  const unsigned nb = (unsigned)div_up(s->ncoord, kBlock);
+  if (s->fp32)
+    k_rec_put<float><<<nb, kBlock, 0, s->stream>>>(static_cast<AsyncRec<float> *>(s->rec),
+                                                   s->ncoord, s->x, s->gref);
+  else
+    k_rec_put<double><<<nb, kBlock, 0, s->stream>>>(static_cast<AsyncRec<double> *>(s->rec),
+                                                    s->ncoord, s->x, nullptr);
+  HIP_TRY(hipGetLastError());
+  s->rec_cur = true;
+  return ACDC_OK;
+}
+acdc_status ensure_x(acdc_solver *s) {
+  if (!s->async || s->x_cur) return ACDC_OK;
+  const unsigned nb = (unsigned)div_up(s->ncoord, kBlock);
+  if (s->fp32)
+    k_rec_get_x<float><<<nb, kBlock, 0, s->stream>>>(static_cast<AsyncRec<float> *>(s->rec),
+                                                     s->ncoord, s->x);
+  else
+    k_rec_get_x<double><<<nb, kBlock, 0, s->stream>>>(static_cast<AsyncRec<double> *>(s->rec),
+                                                      s->ncoord, s->x);
+  HIP_TRY(hipGetLastError());
+  s->x_cur = true;
+  return ACDC_OK;
+}
 
 template <typename T, int G, int LOSS>
 struct LaunchSerial {
@@ -380,6 +420,9 @@ struct LaunchEvalCoord {
 // r64 = A x - b (SVM: w) from the nonzero coordinates; FP32 path: then
 // gref = A^T r64 with the evaluation statistics in stats[0..7], dr = 0.
 acdc_status refresh(acdc_solver *s) {
+  acdc_status st = ensure_x(s);
+  if (st != ACDC_OK) return st;
+  if (s->fp32) s->rec_cur = false;  // the anchor changes
   const unsigned rb = (unsigned)div_up(s->nres, kBlock);
   k_res_init<<<rb, kBlock, 0, s->stream>>>(s->nres, s->loss == kSvm ? nullptr : s->rhs, s->r64);
   dispatch3<LaunchRefreshScatter>(s, s->eval_lanes, s->r64);
@@ -388,7 +431,8 @@ acdc_status refresh(acdc_solver *s) {
     k_fill<float><<<rb, kBlock, 0, s->stream>>>(static_cast<float *>(s->res), s->nres, 0.0f);
   }
   HIP_TRY(hipGetLastError());
-  s->updates_since_refresh = 0;
+  s->steps_since_refresh = 0;
+  s->stats_cur = s->fp32;
   return ACDC_OK;
 }
 
@@ -541,6 +585,12 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
   if ((st = dalloc_t(s, &s->partial, (size_t)kEvalBlocks * kStats)) != ACDC_OK) return bail(st);
   if ((st = dalloc_t(s, &s->stats, 2 * kStats)) != ACDC_OK) return bail(st);
   if ((st = dalloc_t(s, &s->gsync, 1)) != ACDC_OK) return bail(st);
+  if (set->step_mode == ACDC_STEP_ASYNC) {
+    const size_t rsz = s->fp32 ? sizeof(AsyncRec<float>) : sizeof(AsyncRec<double>);
+    const size_t esz = s->fp32 ? sizeof(Entry<float>) : sizeof(Entry<double>);
+    if ((st = dalloc(s, &s->rec, (size_t)s->ncoord * rsz)) != ACDC_OK) return bail(st);
+    if ((st = dalloc(s, &s->ent, (size_t)p->nnz * esz)) != ACDC_OK) return bail(st);
+  }
 
   hipError_t e = hipMemcpy(s->ptr, p->cptr.data(), p->cptr.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess && p->nnz > 0)
@@ -580,6 +630,24 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
                        "of the persistent kernel"));
   }
   dispatch3<LaunchLipschitz>(s, s->eval_lanes);
+  if (s->async) {
+    const unsigned nb = (unsigned)div_up(s->ncoord, kBlock);
+    const unsigned eb = (unsigned)std::max<int64_t>(1, div_up(s->nnz, kBlock));
+    const double *y = loss == kSvm ? s->rhs : nullptr;
+    if (s->fp32) {
+      k_rec_init<float><<<nb, kBlock, 0, s->stream>>>(static_cast<AsyncRec<float> *>(s->rec),
+                                                      s->ncoord, s->ptr, s->L, y);
+      k_pack_entries<float><<<eb, kBlock, 0, s->stream>>>(
+          s->idx, static_cast<const float *>(s->val), s->nnz, static_cast<Entry<float> *>(s->ent));
+    } else {
+      k_rec_init<double><<<nb, kBlock, 0, s->stream>>>(static_cast<AsyncRec<double> *>(s->rec),
+                                                       s->ncoord, s->ptr, s->L, y);
+      k_pack_entries<double><<<eb, kBlock, 0, s->stream>>>(
+          s->idx, static_cast<const double *>(s->val), s->nnz,
+          static_cast<Entry<double> *>(s->ent));
+    }
+    s->rec_cur = false;
+  }
   if ((st = refresh(s)) != ACDC_OK) return bail(st);
   if (hipStreamSynchronize(s->stream) != hipSuccess)
     return bail(fail(ACDC_ERR_HIP, "acdc_solver_create: setup kernels failed"));
@@ -597,19 +665,24 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
   const uint64_t tau = (uint64_t)s->set.tau;
   const uint64_t nsteps = (n_updates + tau - 1) / tau;
   if (nsteps == 0) return ACDC_OK;
-  const uint64_t period =
-      s->fp32 && s->set.refresh_period > 0 ? (uint64_t)s->set.refresh_period * (uint64_t)s->ncoord : 0;
+  // refresh every refresh_period epochs of ceil(n / tau) steps each, so that
+  // runs of whole epochs end exactly on a refresh
+  const uint64_t period = s->fp32 && s->set.refresh_period > 0
+                              ? (uint64_t)s->set.refresh_period * (((uint64_t)s->ncoord + tau - 1) / tau)
+                              : 0;
   acdc_status st;
   if ((st = timed_begin(s)) != ACDC_OK) return st;
   uint64_t done = 0;
   while (done < nsteps) {
     uint64_t chunk = nsteps - done;
     if (period) {
-      const uint64_t left = period > s->updates_since_refresh ? period - s->updates_since_refresh : 1;
-      chunk = std::min(chunk, (left + tau - 1) / tau);
+      const uint64_t left = period > s->steps_since_refresh ? period - s->steps_since_refresh : 1;
+      chunk = std::min(chunk, left);
     }
+    if ((st = ensure_rec(s)) != ACDC_OK) return st;
     if ((st = step_mark(s)) != ACDC_OK) return st;
     if (s->async) {
+      s->x_cur = false;
       // equal launches of at most kAsyncPerLaunch updates per worker
       const uint64_t nl = (chunk + kAsyncPerLaunch - 1) / kAsyncPerLaunch;
       const uint64_t per = (chunk + nl - 1) / nl;
@@ -627,11 +700,12 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
       s->step_launches += 2 * chunk;
     }
     if ((st = step_mark(s)) != ACDC_OK) return st;
+    s->stats_cur = false;
     s->step += chunk;
     s->updates += chunk * tau;
-    s->updates_since_refresh += chunk * tau;
+    s->steps_since_refresh += chunk;
     done += chunk;
-    if (period && s->updates_since_refresh >= period)
+    if (period && s->steps_since_refresh >= period)
       if ((st = refresh(s)) != ACDC_OK) return st;
   }
   HIP_TRY(hipGetLastError());
@@ -663,8 +737,11 @@ extern "C" acdc_status acdc_report_get(acdc_solver *s, acdc_report *out) {
   acdc_status st;
   if ((st = timed_begin(s)) != ACDC_OK) return st;
   if (s->fp32) {
-    if ((st = refresh(s)) != ACDC_OK) return st;  // stats come with the anchor
+    // stats come with the anchor; a refresh that no update has followed yet
+    // (the periodic one at the end of a run) already holds them
+    if (!s->stats_cur && (st = refresh(s)) != ACDC_OK) return st;
   } else {
+    if ((st = ensure_x(s)) != ACDC_OK) return st;
     dispatch3<LaunchEvalCoord>(s, s->eval_lanes, nullptr);
   }
   const int64_t rblocks = std::min(kEvalBlocks, div_up(s->nres, kBlock));
@@ -720,6 +797,8 @@ extern "C" acdc_status acdc_report_get(acdc_solver *s, acdc_report *out) {
 extern "C" acdc_status acdc_get_x(acdc_solver *s, double *x, int64_t n) {
   if (!s || !x) return fail(ACDC_ERR_INVALID, "acdc_get_x: null argument");
   if (n != s->ncoord) return fail(ACDC_ERR_INVALID, "acdc_get_x: length must equal n");
+  acdc_status st = ensure_x(s);
+  if (st != ACDC_OK) return st;
   HIP_TRY(hipStreamSynchronize(s->stream));
   HIP_TRY(hipMemcpy(x, s->x, n * 8, hipMemcpyDeviceToHost));
   return ACDC_OK;
@@ -730,6 +809,8 @@ extern "C" acdc_status acdc_set_x(acdc_solver *s, const double *x, int64_t n) {
   if (n != s->ncoord) return fail(ACDC_ERR_INVALID, "acdc_set_x: length must equal n");
   HIP_TRY(hipStreamSynchronize(s->stream));
   HIP_TRY(hipMemcpy(s->x, x, n * 8, hipMemcpyHostToDevice));
+  s->x_cur = true;
+  s->rec_cur = false;
   acdc_status st = refresh(s);
   if (st != ACDC_OK) return st;
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -773,12 +854,14 @@ extern "C" acdc_status acdc_set_lambda(acdc_solver *s, double lambda) {
       k_scale<double><<<rb, kBlock, 0, s->stream>>>(s->r64, s->nres, f);
       k_scale<double><<<(unsigned)div_up(s->ncoord, kBlock), kBlock, 0, s->stream>>>(
           s->gref, s->ncoord, f);
+      s->rec_cur = false;
     } else {
       k_scale<double><<<rb, kBlock, 0, s->stream>>>(s->r64, s->nres, f);
     }
     HIP_TRY(hipGetLastError());
   }
   s->set.lambda = lambda;
+  s->stats_cur = false;
   return ACDC_OK;
 }
 
@@ -841,7 +924,7 @@ extern "C" acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps
       (void)hipEventRecord(ev[3 * k + 2], s->stream);
       s->step += 1;
       s->updates += (uint64_t)s->set.tau;
-      s->updates_since_refresh += (uint64_t)s->set.tau;
+      s->steps_since_refresh += 1;
     }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);

@@ -80,7 +80,8 @@ typedef struct acdc_settings {
   int32_t lanes_per_coord; /* 0 = auto, else 8, 16, 32 or 64 lanes per
                               sampled coordinate */
   int32_t refresh_period;  /* FP32 path: FP64 residual refresh every this
-                              many epochs inside acdc_run_updates (0 = off) */
+                              many epochs of ceil(n/tau) steps inside
+                              acdc_run_updates (0 = off) */
   int32_t launch_mode;     /* acdc_launch_mode: how tau > 1 steps execute */
 } acdc_settings;
 
