@@ -97,7 +97,8 @@ class BatchSettings(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("precision", ctypes.c_int32),
                 ("lanes_per_coord", ctypes.c_int32), ("refresh_period", ctypes.c_int32),
                 ("max_epochs", ctypes.c_int32), ("gap_tol", ctypes.c_double),
-                ("use_graph", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("step_mode", ctypes.c_int32),
+                ("tau", ctypes.c_int32)]
 
 
 class _BatchResult(ctypes.Structure):
@@ -393,7 +394,7 @@ class Batch:
 
     def __init__(self, problems, lambdas, *, seed=1, precision=PREC_FP32_REFRESH,
                  lanes_per_coord=0, refresh_period=5, max_epochs=200, gap_tol=1e-6,
-                 use_graph=True):
+                 use_graph=True, step_mode=STEP_SYNC, tau=0):
         lam = _f64(lambdas)
         if lam.ndim != 2 or lam.shape[0] != len(problems):
             raise ValueError("lambdas must be (instances, path_len)")
@@ -402,7 +403,7 @@ class Batch:
         self.n = [p.n for p in problems]
         arr = (_P * self.count)(*[p._h for p in problems])
         st = BatchSettings(seed, precision, lanes_per_coord, refresh_period, max_epochs, gap_tol,
-                           int(bool(use_graph)))
+                           int(bool(use_graph)), step_mode, tau)
         h = _P()
         _check(lib.acdc_batch_create(arr, self.count, _ptr(lam, ctypes.c_double), self.path_len,
                                      ctypes.byref(st), ctypes.byref(h)))

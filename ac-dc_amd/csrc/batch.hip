@@ -46,6 +46,7 @@ constexpr int kSlotsPerGroup = 2;
 struct acdc_batch {
   acdc_batch_settings set{};
   bool fp32 = true;
+  bool async = false;  // ACDC_STEP_ASYNC inside instances
   int lanes = 16;
   int32_t count = 0, path_len = 0;
   std::vector<int64_t> n, m, xoff;
@@ -95,23 +96,27 @@ void destroy_batch(acdc_batch *b) {
 constexpr int regs_for(int lanes) { return lanes == 64 ? 2 : 4; }
 
 template <typename T, int G, bool LDSR>
-const void *batch_fn() {
+const void *batch_fn(bool async) {
+  if (async) return reinterpret_cast<const void *>(&k_batch_async<T, G, regs_for(G), LDSR>);
   return reinterpret_cast<const void *>(&k_batch_lasso<T, G, kSlotsPerGroup, regs_for(G), LDSR>);
 }
 
 template <typename T, bool LDSR>
-const void *batch_fn_lanes(int lanes) {
+const void *batch_fn_lanes(int lanes, bool async) {
   switch (lanes) {
-    case 8: return batch_fn<T, 8, LDSR>();
-    case 16: return batch_fn<T, 16, LDSR>();
-    case 32: return batch_fn<T, 32, LDSR>();
-    default: return batch_fn<T, 64, LDSR>();
+    case 8: return batch_fn<T, 8, LDSR>(async);
+    case 16: return batch_fn<T, 16, LDSR>(async);
+    case 32: return batch_fn<T, 32, LDSR>(async);
+    default: return batch_fn<T, 64, LDSR>(async);
   }
 }
 
 const void *batch_kernel(const acdc_batch *b, bool lds) {
-  if (b->fp32) return lds ? batch_fn_lanes<float, true>(b->lanes) : batch_fn_lanes<float, false>(b->lanes);
-  return lds ? batch_fn_lanes<double, true>(b->lanes) : batch_fn_lanes<double, false>(b->lanes);
+  if (b->fp32)
+    return lds ? batch_fn_lanes<float, true>(b->lanes, b->async)
+               : batch_fn_lanes<float, false>(b->lanes, b->async);
+  return lds ? batch_fn_lanes<double, true>(b->lanes, b->async)
+             : batch_fn_lanes<double, false>(b->lanes, b->async);
 }
 
 // enqueue reset + the two persistent launches on b->stream
@@ -165,6 +170,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: null argument or empty batch");
   if (set->precision != ACDC_PREC_FP64 && set->precision != ACDC_PREC_FP32_REFRESH)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: unknown precision");
+  if (set->step_mode != ACDC_STEP_SYNC && set->step_mode != ACDC_STEP_ASYNC)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: unknown step mode");
   if (set->refresh_period < 1 || set->max_epochs < 0)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: refresh_period >= 1 and max_epochs >= 0");
   if (set->lanes_per_coord != 0 && set->lanes_per_coord != 8 && set->lanes_per_coord != 16 &&
@@ -191,6 +198,7 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   };
   b->set = *set;
   b->fp32 = set->precision == ACDC_PREC_FP32_REFRESH;
+  b->async = set->step_mode == ACDC_STEP_ASYNC;
   b->count = count;
   b->path_len = path_len;
   int64_t tot_nnz = 0, tot_n = 0, tot_m = 0;
@@ -201,7 +209,13 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   }
   b->lanes = set->lanes_per_coord ? set->lanes_per_coord
                                   : (tot_nnz <= 8 * tot_n ? 8 : (tot_nnz <= 24 * tot_n ? 16 : 32));
-  const int32_t slots = kSlotsPerGroup * (kBatchBlock / b->lanes);
+  // tau per instance: the block's coordinate slots per synchronous step, or
+  // its lane groups as asynchronous workers
+  const int32_t cap = (b->async ? 1 : kSlotsPerGroup) * (kBatchBlock / b->lanes);
+  if (set->tau < 0 || set->tau > cap)
+    return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: tau must lie in [0, " +
+                                            std::to_string(cap) + "] for these lanes and mode"));
+  const int32_t slots = set->tau ? set->tau : cap;
   const size_t tsz = b->fp32 ? 4 : 8;
   const size_t esz = b->fp32 ? sizeof(Entry<float>) : sizeof(Entry<double>);
   const int64_t lds_rows = (int64_t)(kLdsBytes / tsz);

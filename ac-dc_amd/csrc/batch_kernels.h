@@ -181,7 +181,7 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
   __syncthreads();
   // r += x_j a_j over the nonzero x_j: a 16-lane group per column
   for (int j = grp; j < I.n; j += NG) {
-    const double xj = I.cs[j].x;
+    const double xj = ld_coherent(&I.cs[j].x);  // async workers add to x atomically
     if (xj == 0.0) continue;
     const int32_t p0 = I.cs[j].p0, len = I.cs[j].len;
     for (int32_t q = lane; q < len; q += G) {
@@ -207,13 +207,13 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
     ga = group_sum<G>(ga);
     gb = group_sum<G>(gb);
     if (lane == 0) {
-      const double xa = I.cs[j0].x;
+      const double xa = ld_coherent(&I.cs[j0].x);
       I.cs[j0].gref = ga;
       ginf = fmax(ginf, fabs(ga));
       x1 += fabs(xa);
       nz += xa != 0.0;
       if (on1) {
-        const double xb = I.cs[j1].x;
+        const double xb = ld_coherent(&I.cs[j1].x);
         I.cs[j1].gref = gb;
         ginf = fmax(ginf, fabs(gb));
         x1 += fabs(xb);
@@ -362,6 +362,143 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
         a.results[(int64_t)i * a.path_len + l] = res;
       }
       __syncthreads();  // s_res is rewritten by the next refresh
+    }
+  }
+}
+
+// ------------------------------------------------- asynchronous instances
+// The multicore executor's semantics inside one block: the block's NG =
+// kBatchBlock/G lane groups are I.tau workers that each draw coordinates of
+// the instance with replacement from their own stream (async_coord keyed by
+// the instance seed and the group) and apply updates with no barrier between
+// them -- LDS atomics on the residual correction, atomic adds on x_j.  Only
+// the FP64 refresh, every refresh_period epochs, stops all workers (one
+// block barrier).  An update reads its 32-byte record two updates ahead and
+// its entries one update ahead, so its critical path is the LDS gather plus
+// the agent-scope load of x_j (agent scope: the L1 does not hold x_j across
+// the other workers' atomic updates).
+template <typename T, int G, int R, bool LDSR>
+__device__ __forceinline__ void batch_async_updates(const BatchInstance &I, double lambda,
+                                                    uint64_t k0, uint64_t count, T *lds) {
+  const int lane = threadIdx.x & (G - 1);
+  const uint32_t w = threadIdx.x / G;
+  if ((int32_t)w >= I.tau || count == 0) return;
+  const Entry<T> *ent = static_cast<const Entry<T> *>(I.ent);
+  T *dr = dr_ptr<T, LDSR>(I, lds);
+  StepScalars sc;
+  sc.lambda = lambda;
+  sc.lambda_n = 0;
+  sc.inv_lambda_n = 0;
+  const bool serial = I.tau == 1;
+  struct Slot {
+    uint32_t j;
+    CoordRec r;
+    int32_t ci[R];
+    T cv[R];
+  };
+  auto record = [&](uint64_t k, Slot &c) {
+    c.j = async_coord(I.seed, w, k0 + k, (uint32_t)I.n);
+    c.r = I.cs[c.j];
+  };
+  auto entries = [&](Slot &c) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t q = lane + r * G;
+      if (q < c.r.len) {
+        const Entry<T> e = ent[c.r.p0 + q];
+        c.ci[r] = e.row;
+        c.cv[r] = e.val;
+      } else {
+        c.ci[r] = 0;
+        c.cv[r] = (T)0;
+      }
+    }
+  };
+  auto gather = [&](int32_t row) -> T {
+    return LDSR ? dr[row] : ld_coherent(&dr[row]);
+  };
+  auto update = [&](uint64_t k, Slot &cur, Slot &nxt, Slot &ahead) {
+    if (k + 1 < count) entries(nxt);
+    if (k + 2 < count) record(k + 2, ahead);
+    const double xj = ld_coherent(&I.cs[cur.j].x);
+    T acc = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (lane + r * G < cur.r.len) acc += cur.cv[r] * gather(cur.ci[r]);
+    for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
+      const Entry<T> e = ent[cur.r.p0 + q];
+      acc += e.val * gather(e.row);
+    }
+    acc = group_sum<G>(acc);
+    double c;
+    const double d = prox_delta<kLasso>(cur.r.gref + (double)acc, xj, cur.r.L, 0.0, sc, &c);
+    if (lane == 0 && d != 0.0) unsafeAtomicAdd(&I.cs[cur.j].x, d);
+    if (c != 0.0) {
+      const T ct = (T)c;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (lane + r * G < cur.r.len) add_to<T, LDSR>(&dr[cur.ci[r]], ct * cur.cv[r]);
+      for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
+        const Entry<T> e = ent[cur.r.p0 + q];
+        add_to<T, LDSR>(&dr[e.row], ct * e.val);
+      }
+    }
+    if (serial)  // one worker: the serial method, each update sees the last
+      asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+    cur.j = ahead.j;
+    cur.r = ahead.r;
+  };
+  Slot s0, s1, ahead;
+  record(0, s0);
+  entries(s0);
+  if (count > 1) record(1, s1);
+  for (uint64_t k = 0; k < count; k += 2) {
+    update(k, s0, s1, ahead);
+    if (k + 1 < count) update(k + 1, s1, s0, ahead);
+  }
+}
+
+template <typename T, int G, int R, bool LDSR>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
+  __shared__ double sh[kRefreshStats * kBatchWaves];
+  __shared__ BatchResult s_res;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= a.count) break;
+    const int i = a.order[item];
+    const BatchInstance *Ip = a.inst + i;
+    const BatchInstance I = *Ip;
+    uint64_t k = 0;  // updates per worker so far
+    for (int l = 0; l < a.path_len; ++l) {
+      const double lambda = a.lambdas[(int64_t)i * a.path_len + l];
+      batch_refresh<T, LDSR>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+      double gap = s_res.rel_gap;
+      int epochs = 0;
+      while (gap > a.gap_tol && epochs < a.max_epochs) {
+        const int ep = min(a.refresh_period, a.max_epochs - epochs);
+        const uint64_t per = ((uint64_t)I.n * (uint64_t)ep + I.tau - 1) / I.tau;
+        batch_async_updates<T, G, R, LDSR>(I, lambda, k, per, lds);
+        __syncthreads();  // every worker's updates are in before the refresh
+        k += per;
+        epochs += ep;
+        batch_refresh<T, LDSR>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+        gap = s_res.rel_gap;
+      }
+      if (threadIdx.x == 0) {
+        BatchResult res = s_res;
+        res.lambda = lambda;
+        res.epochs = epochs;
+        res.converged = gap <= a.gap_tol;
+        res.updates = k * (uint64_t)I.tau;
+        a.results[(int64_t)i * a.path_len + l] = res;
+      }
+      __syncthreads();
     }
   }
 }

@@ -231,6 +231,13 @@ typedef struct acdc_batch_settings {
   int32_t max_epochs;      /* per lambda */
   double gap_tol;          /* relative duality gap target (< 0: always max_epochs) */
   int32_t use_graph;       /* capture the run in a HIP graph at create */
+  int32_t step_mode;       /* acdc_step_mode inside each instance: SYNC = tau-nice
+                              steps with block barriers (replays the oracle);
+                              ASYNC = the block's lane groups as unsynchronised
+                              workers with LDS atomics */
+  int32_t tau;             /* 0 = the block's capacity (SYNC: 2 slots per lane
+                              group; ASYNC: one worker per lane group), else
+                              at most that, per instance at most n */
 } acdc_batch_settings;
 
 typedef struct acdc_batch_result {

@@ -98,3 +98,48 @@ def test_batch_graph_replays_match_eager(acdc, small_batch):
             assert np.max(np.abs(x - ref[i])) <= 1e-10 * max(1.0, np.max(np.abs(ref[i]))), (rep, i)
     eager.close()
     g.close()
+
+
+def test_batch_async_one_worker_is_serial(acdc, orc, small_batch):
+    """ASYNC instances with tau = 1: the one worker's stream (async_coord
+    keyed by seed + i, worker 0) replayed by the serial oracle over the
+    warm-started path, at FP64 (LDS and global residuals both)."""
+    ds = small_batch
+    lams = lambdas_of(acdc, ds, 3, 0.1)
+    probs = [d.problem() for d in ds]
+    b = acdc.Batch(probs, lams, seed=70, precision=acdc.PREC_FP64, refresh_period=1,
+                   max_epochs=2, gap_tol=-1.0, use_graph=False, step_mode=acdc.STEP_ASYNC, tau=1)
+    b.run()
+    res = b.results()
+    assert (res["epochs"] == 2).all()
+    for i, d in enumerate(ds):
+        assert b.info(i)[0] == 1
+        o = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[i][0])
+        k = 0
+        for lam in lams[i]:
+            o.lam = float(lam)
+            o.run_list(orc.async_coords(70 + i, 0, k, 2 * d.n, d.n))
+            k += 2 * d.n
+        x = b.x(i)
+        err = np.max(np.abs(x - o.x)) / max(1.0, np.max(np.abs(o.x)))
+        assert err < 1e-10, (i, err)
+        assert res[i, -1]["primal"] == pytest.approx(o.eval()["primal"], rel=1e-10)
+    b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batch_async_workers_reach_fp64_optimum(acdc, orc, small_batch, prec):
+    ds = small_batch
+    lams = lambdas_of(acdc, ds, 4, 0.05)
+    probs = [d.problem() for d in ds]
+    precision = acdc.PREC_FP32_REFRESH if prec == "fp32" else acdc.PREC_FP64
+    b = acdc.Batch(probs, lams, seed=9, precision=precision, refresh_period=5, max_epochs=400,
+                   gap_tol=1e-7, use_graph=True, step_mode=acdc.STEP_ASYNC)
+    b.run()
+    res = b.results()
+    assert res["converged"].all(), res["rel_gap"]
+    for i, d in enumerate(ds):
+        e = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[i, -1]).eval(b.x(i))
+        assert e["rel_gap"] <= 1.01e-7
+        assert res[i, -1]["primal"] == pytest.approx(e["primal"], rel=1e-9)
+    b.close()
