@@ -207,8 +207,15 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     tot_n += problems[i]->ncoord;
     tot_m += problems[i]->nres;
   }
-  b->lanes = set->lanes_per_coord ? set->lanes_per_coord
-                                  : (tot_nnz <= 8 * tot_n ? 8 : (tot_nnz <= 24 * tot_n ? 16 : 32));
+  // lanes per coordinate (measured on the 200k-column instances of the batched
+  // Lasso workload: asynchronous workers peak at 16 lanes = 64 workers per
+  // block up to ~48 entries per column on average)
+  if (set->lanes_per_coord)
+    b->lanes = set->lanes_per_coord;
+  else if (b->async)
+    b->lanes = tot_nnz <= 48 * tot_n ? 16 : 32;
+  else
+    b->lanes = tot_nnz <= 8 * tot_n ? 8 : (tot_nnz <= 24 * tot_n ? 16 : 32);
   // tau per instance: the block's coordinate slots per synchronous step, or
   // its lane groups as asynchronous workers
   const int32_t cap = (b->async ? 1 : kSlotsPerGroup) * (kBatchBlock / b->lanes);

@@ -67,6 +67,85 @@ def throughput(solver, n, epochs):
             "launches": r1["step_launches"] - r0["step_launches"]}
 
 
+def batched(acdc, args):
+    """The batched workload: 256 Lasso instances of uneven sizes (m log-uniform
+    in [1k,50k], n in [5k,200k], column nnz truncated Zipf(1.1) on [3,100]),
+    each along a 32-value warm-started path lambda_max -> 1e-3 lambda_max,
+    stopping at FP64 relative gap <= 1e-6 or the epoch cap per lambda; one
+    graph-captured persistent launch (FP32 + FP64 refresh every 5 epochs,
+    asynchronous workers inside each instance).  Beside it, the plain FP64
+    path (synchronous per-step launches, one instance at a time) on sampled
+    instances over the first lambdas of their paths, against the batched
+    kernel on the same instances and lambdas."""
+    import numpy as np
+    t = time.perf_counter()
+    ds = acdc.config_batch_c5(count=args.batch_count)
+    nnz = np.array([d.nnz for d in ds])
+    log(f"[bench] batched: generated {len(ds)} instances, nnz {nnz.sum()} in "
+        f"{time.perf_counter() - t:.1f}s")
+    probs = [d.problem() for d in ds]
+    lams = np.stack([acdc.lambda_path(d.lambda_max, args.batch_path, 1e-3) for d in ds])
+    kw = dict(seed=5, precision=acdc.PREC_FP32_REFRESH, refresh_period=5,
+              max_epochs=args.batch_max_epochs, gap_tol=args.gap_target,
+              step_mode=acdc.STEP_ASYNC)
+    b = acdc.Batch(probs, lams, use_graph=True, **kw)  # uploads, captures the graph
+    t0 = time.perf_counter()
+    ms = b.run()
+    wall = time.perf_counter() - t0
+    r = b.results()
+    upd = float(r["updates"][:, -1].astype(np.float64).sum())
+    out = {"workload": f"{len(ds)} lasso instances x {args.batch_path} lambdas",
+           "nnz": int(nnz.sum()), "max_epochs_per_lambda": args.batch_max_epochs,
+           "gap_tol": args.gap_target, "device_ms": ms, "wall_s": wall, "updates": upd,
+           "value": upd / wall, "unit": "coord updates/s",
+           "pairs_converged": float(r["converged"].mean()),
+           "worst_rel_gap": float(r["rel_gap"].max()),
+           "tau_per_instance": int(b.info(0)[0])}
+    log(f"[bench] batched: {ms:.0f} ms device, {upd:.3e} updates = {upd / wall:.3e} upd/s, "
+        f"{out['pairs_converged']:.3f} of (instance, lambda) pairs converged")
+    b.close()
+    # plain FP64 vs the batched kernel on sampled instances (quartiles of nnz)
+    order = np.argsort(nnz)
+    sample = [int(order[int(q * (len(ds) - 1))]) for q in (0.25, 0.5, 0.75)]
+    P = min(args.batch_sample_path, args.batch_path)
+    rows = []
+    for i in sample:
+        d, p = ds[i], probs[i]
+        bi = acdc.Batch([p], lams[i:i + 1, :P], use_graph=False, **kw)
+        bi.run()
+        tb = bi.run() * 1e-3
+        ri = bi.results()
+        tau = int(bi.info(0)[0])
+        bi.close()
+        s = acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=float(lams[i, 0]), tau=tau, seed=5,
+                        precision=acdc.PREC_FP64, launch_mode=acdc.LAUNCH_PER_STEP)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for l in range(P):
+            s.set_lambda(float(lams[i, l]))
+            ep, rep = 0, s.report()
+            while rep["rel_gap"] > args.gap_target and ep < args.batch_max_epochs:
+                for _ in range(5):
+                    s.run_updates(d.n)
+                ep += 5
+                rep = s.report()
+        tp = time.perf_counter() - t0
+        up = s.report()["updates"]
+        s.close()
+        rows.append({"instance": i, "m": d.m, "n": d.n, "nnz": d.nnz, "lambdas": P,
+                     "batched_s": tb, "batched_updates": int(ri["updates"][0, -1]),
+                     "fp64_plain_s": tp, "fp64_plain_updates": int(up),
+                     "speedup": tp / tb})
+        log(f"[bench] batched sample {i} ({d.m}x{d.n}): batched {tb:.3f}s vs plain fp64 "
+            f"{tp:.3f}s = {tp / tb:.1f}x")
+    out["fp64_plain_sampled"] = rows
+    # the whole workload at the plain path's measured cost per update
+    cpu = [row["fp64_plain_s"] / max(1, row["fp64_plain_updates"]) for row in rows]
+    out["fp64_plain_estimate_s"] = float(np.mean(cpu) * upd)
+    out["speedup_vs_fp64_plain_estimate"] = out["fp64_plain_estimate_s"] / wall
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +161,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttg", action="store_true", help="skip the time-to-gap runs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-path", type=int, default=32)
+    ap.add_argument("--batch-max-epochs", type=int, default=100)
+    ap.add_argument("--batch-sample-path", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -181,6 +265,9 @@ def main():
             result["time_to_gap"]["speedup_vs_fp64_plain"] = (
                 result["fp64_plain"]["time_to_gap"]["seconds"] / result["time_to_gap"]["seconds"])
     s64.close()
+
+    if not args.no_batch:
+        result["batched"] = batched(acdc, args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
