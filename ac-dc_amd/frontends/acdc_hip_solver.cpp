@@ -33,7 +33,7 @@ static void usage() {
   std::cerr << "usage: acdc_hip_solver -A <data> [-b <b file>] -l <lambda> [-t tau] [-f loss]\n"
                "                       [-I updates per bulk] [-c bulks] [-i 0|1] [-r log]\n"
                "                       [-s seed] [-p precision] [-R refresh epochs] [-L lanes]\n"
-               "                       [-P launch mode]\n";
+               "                       [-P launch mode] [-M step mode: 0 sync, 1 async]\n";
 }
 
 int main(int argc, char *argv[]) {
@@ -42,7 +42,7 @@ int main(int argc, char *argv[]) {
   std::string matrixA, vectorB, logPath;
   double lambda = 0;
   int c;
-  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:s:p:R:L:P:h")) != -1) {
+  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:s:p:R:L:P:M:h")) != -1) {
     switch (c) {
       case 'A': matrixA = optarg; break;
       case 'b': vectorB = optarg; break;
@@ -64,6 +64,7 @@ int main(int argc, char *argv[]) {
       case 'R': settings.hipRefreshPeriod = std::atoi(optarg); break;
       case 'L': settings.hipLanesPerCoord = std::atoi(optarg); break;
       case 'P': settings.hipLaunchMode = std::atoi(optarg); break;
+      case 'M': settings.hipStepMode = std::atoi(optarg); break;
       default: usage(); return c == 'h' ? 0 : 2;
     }
   }

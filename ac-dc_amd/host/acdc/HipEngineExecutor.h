@@ -99,7 +99,7 @@ class HipEngineExecutor : public AbstractEngineExecutor<L, D> {
       s.beta_mode = ACDC_BETA_OMEGA;
     }
     s.precision = _settings->hipPrecision;
-    s.step_mode = ACDC_STEP_SYNC;
+    s.step_mode = _settings->hipStepMode;
     s.lanes_per_coord = _settings->hipLanesPerCoord;
     s.refresh_period = _settings->hipRefreshPeriod;
     s.launch_mode = _settings->hipLaunchMode;

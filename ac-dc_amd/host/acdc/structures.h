@@ -68,6 +68,8 @@ class OptimizationSettings {
   int hipRefreshPeriod = 5;                   // FP32 path: FP64 refresh every k epochs
   int hipLanesPerCoord = 0;                   // 0 = auto
   int hipLaunchMode = ACDC_LAUNCH_AUTO;
+  int hipStepMode = ACDC_STEP_SYNC;           // ACDC_STEP_ASYNC: the multicore executor's
+                                              // unsynchronised workers (tau = totalThreads)
 };
 
 class OptimizationStatistics {

@@ -34,6 +34,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic():
+    """HBM bytes per k_async launch on this workload from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by a copy of known size),
+    or None when the summary is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "lasso_c2_async_traffic.md")
+    try:
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.]+)", open(path).read(), re.M)
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
+
+
 def time_to_gap(solver, n, target, check_every, max_epochs):
     """Wall time from x = 0 to rel gap <= target: one run_updates per epoch,
     the FP64 gap every check_every epochs (evaluations included)."""
@@ -221,7 +234,7 @@ def main():
                                 "fp64 refresh every 5 epochs",
                    "step": "one epoch = n coordinate updates, from x = 0"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "kernel": "k_async<float,G,0,4,true>", "launch_ms": launch_ms,
                      "updates_per_launch": upd_per_launch,
                      "bytes_per_update": bytes_per_update, "bytes_per_launch": bytes_per_launch},
