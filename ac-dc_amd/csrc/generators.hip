@@ -393,8 +393,9 @@ extern "C" acdc_status acdc_gen_ttd(int32_t cells, acdc_dataset **out) {
             for (int c = 0; c < 3; ++c) {
               if (dv[c] == 0) continue;
               const double beta = dv[c] / len2;
-              if (freeid[a] >= 0) ent.push_back({(int32_t)(c * nfree + freeid[a]), -beta});
-              if (freeid[b] >= 0) ent.push_back({(int32_t)(c * nfree + freeid[b]), beta});
+              // compression positive: +d on the lower end a, -d on the upper end b
+              if (freeid[a] >= 0) ent.push_back({(int32_t)(c * nfree + freeid[a]), beta});
+              if (freeid[b] >= 0) ent.push_back({(int32_t)(c * nfree + freeid[b]), -beta});
             }
             std::sort(ent.begin(), ent.end());
             for (auto &e : ent) {

@@ -425,8 +425,38 @@ __global__ __launch_bounds__(kAsyncBlock) void k_async(AsyncArgs<T> a, uint64_t 
     acc = group_sum<G>(acc);
     double c;
     const double g = rec_gref(cur.r) + (double)acc;
-    const double d = prox_delta<LOSS>(g, xj, rec_L(cur.r), (double)cur.r.y, a.sc, &c);
-    if (lane == 0 && d != 0.0) unsafeAtomicAdd(&a.rec[cur.j].x, d);
+    if constexpr (LOSS == kLasso) {
+      const double d = prox_delta<LOSS>(g, xj, rec_L(cur.r), (double)cur.r.y, a.sc, &c);
+      if (lane == 0 && d != 0.0) unsafeAtomicAdd(&a.rec[cur.j].x, d);
+    } else {
+      // constrained coordinates (x >= 0, alpha in [0,1]): the new value is
+      // computed from the x_j it replaces, by compare-and-swap, so that two
+      // workers racing on one coordinate cannot leave it infeasible; the
+      // scatter applies the delta of the step that was swapped in
+      double d = 0.0;
+      if (lane == 0) {
+        unsigned long long *px = reinterpret_cast<unsigned long long *>(&a.rec[cur.j].x);
+        double old = xj;
+        for (;;) {
+          double cc;
+          const double dd = prox_delta<LOSS>(g, old, rec_L(cur.r), (double)cur.r.y, a.sc, &cc);
+          const double nw = old + dd;
+          if (nw == old) {  // x_j unchanged; the residual still takes dd
+            d = dd;
+            break;
+          }
+          const unsigned long long prev =
+              atomicCAS(px, __double_as_longlong(old), __double_as_longlong(nw));
+          if (prev == (unsigned long long)__double_as_longlong(old)) {
+            d = dd;
+            break;
+          }
+          old = __longlong_as_double((long long)prev);
+        }
+      }
+      d = __shfl(d, 0, G);
+      c = LOSS == kSvm ? d * (double)cur.r.y * a.sc.inv_lambda_n : d;
+    }
     if (c != 0.0) {
       const T ct = (T)c;
 #pragma unroll

@@ -620,7 +620,14 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     (void)hipGetDevice(&dev);
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
       dispatch3<Occupancy>(s, s->lanes, &per_cu);
-      s->coresident_blocks = (int64_t)per_cu * prop.multiProcessorCount;
+      // at most one 1024-thread block per CU: a 512-block grid at two blocks
+      // per CU (which the occupancy query allows) did not make every block
+      // arrive at the grid barrier within its bound on the MI355X
+      s->coresident_blocks = (int64_t)std::min(per_cu, 1) * prop.multiProcessorCount;
+      // with automatic lanes, narrow the groups until tau slots fit the grid
+      if (!set->lanes_per_coord)
+        while (s->lanes > 8 && div_up(set->tau * s->lanes, kPersistentBlock) > s->coresident_blocks)
+          s->lanes /= 2;
     }
     const int64_t need = div_up(set->tau * s->lanes, kPersistentBlock);
     s->persistent = need <= s->coresident_blocks;

@@ -159,6 +159,67 @@ def batched(acdc, args):
     return out
 
 
+def svm_c3(acdc, args):
+    """Hinge-SVM dual minibatch SDCA on 20,000 samples x 1,350,000 features
+    (lognormal row nnz, median 300): synchronous minibatches b in {512, 2048,
+    8192} (tau = b, ESO beta of b), FP32 + FP64 refresh of w every 5 epochs
+    and the plain FP64 path; updates/s over the first 20 epochs and the FP64
+    relative duality gap after them."""
+    d = acdc.config_svm_c3()
+    p = d.problem()
+    out = {"workload": f"svm dual {d.n} samples x {d.m} features", "nnz": d.nnz,
+           "epochs": 20, "rows": []}
+    for b in (512, 2048, 8192):
+        for prec, name in ((acdc.PREC_FP32_REFRESH, "fp32"), (acdc.PREC_FP64, "fp64")):
+            s = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=b, seed=3,
+                            precision=prec, refresh_period=5)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.run_updates(20 * d.n)
+            s.synchronize()
+            wall = time.perf_counter() - t0
+            r = s.report()
+            out["rows"].append({"b": b, "precision": name, "beta": s.beta,
+                                "updates_per_s": 20 * d.n / wall, "rel_gap": r["rel_gap"],
+                                "accuracy": r["accuracy"]})
+            log(f"[bench] svm b={b} {name}: {20 * d.n / wall:.3e} upd/s, rel gap {r['rel_gap']:.2e}")
+            s.close()
+    return out
+
+
+def ttd_c4(acdc, args):
+    """Truss topology design NNLS on the 40^3-cell ground structure (844,840
+    bars, 201,720 DOFs): tau in {1024, 8192}, FP32 + FP64 refresh
+    (asynchronous workers) and the plain FP64 path (synchronous per-step
+    launches); updates/s over 50 epochs from x = 0 and the FP64 objective and
+    KKT residual after them."""
+    d = acdc.gen_ttd(40)
+    p = d.problem()
+    out = {"workload": f"ttd nnls 40^3 cells, {d.n} bars x {d.m} dofs", "nnz": d.nnz,
+           "epochs": 50, "rows": []}
+    for tau in (1024, 8192):
+        for prec, mode, name in ((acdc.PREC_FP32_REFRESH, acdc.STEP_ASYNC, "fp32 async"),
+                                 (acdc.PREC_FP64, acdc.STEP_SYNC, "fp64 plain")):
+            s = acdc.Solver(p, acdc.LOSS_NNLS, lam=0.0, tau=tau, seed=4, precision=prec,
+                            refresh_period=5, step_mode=mode,
+                            launch_mode=acdc.LAUNCH_PER_STEP if mode == acdc.STEP_SYNC
+                            else acdc.LAUNCH_AUTO)
+            p0 = s.report()["primal"]
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.run_updates(50 * d.n)
+            s.synchronize()
+            wall = time.perf_counter() - t0
+            r = s.report()
+            out["rows"].append({"tau": tau, "path": name, "beta": s.beta,
+                                "updates_per_s": 50 * d.n / wall, "primal": r["primal"],
+                                "primal_at_0": p0, "kkt": r["kkt"]})
+            log(f"[bench] ttd tau={tau} {name}: {50 * d.n / wall:.3e} upd/s, "
+                f"objective {r['primal']:.4e} (from {p0:.4e}), kkt {r['kkt']:.2e}")
+            s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +236,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttg", action="store_true", help="skip the time-to-gap runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-svm-ttd", action="store_true", help="skip the SVM and TTD workloads")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-path", type=int, default=32)
     ap.add_argument("--batch-max-epochs", type=int, default=100)
@@ -281,6 +343,9 @@ def main():
 
     if not args.no_batch:
         result["batched"] = batched(acdc, args)
+    if not args.no_svm_ttd:
+        result["svm"] = svm_c3(acdc, args)
+        result["ttd"] = ttd_c4(acdc, args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -67,9 +67,13 @@ acdc_status acdc_gen_lasso(const acdc_gen_lasso_cfg *cfg, acdc_dataset **out);
 acdc_status acdc_gen_svm(const acdc_gen_svm_cfg *cfg, acdc_dataset **out);
 /* 3D truss ground structure on a cells^3 grid of unit cells: bars between
  * all node pairs at Chebyshev distance 1; the z = 0 face is fixed and its
- * DOFs removed; column = +-d/||d||^2 on the free end DOFs (zero components
- * not stored, rows ordered dof-major: x block, y block, z block of the free
- * nodes); f = -1 on the z DOF of every top-face node. */
+ * DOFs removed; f = -1 on the z DOF of every top-face node.  A bar from node
+ * a to node b = a + d (d lexicographically positive, so b is not below a)
+ * holds +d/||d||^2 on a's free DOFs and -d/||d||^2 on b's: x_j >= 0 is the
+ * bar's compressive force.  (TTDGenerator.h:186-216 orients bars the other
+ * way, which suits its sign-free L1 formulation; with x >= 0 that
+ * orientation makes x = 0 optimal.)  Zero components are not stored; rows
+ * are dof-major: x block, y block, z block of the free nodes. */
 acdc_status acdc_gen_ttd(int32_t cells, acdc_dataset **out);
 
 acdc_status acdc_dataset_info_get(const acdc_dataset *d, acdc_dataset_info *out);
