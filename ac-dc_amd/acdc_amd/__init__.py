@@ -104,7 +104,7 @@ class BatchSettings(ctypes.Structure):
 class _BatchResult(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ("lambda_", "primal", "dual", "rel_gap", "nnz_x")] + \
                [("epochs", ctypes.c_int32), ("converged", ctypes.c_int32),
-                ("updates", ctypes.c_uint64)]
+                ("updates", ctypes.c_uint64), ("device_us", ctypes.c_double)]
 
 
 def _load() -> ctypes.CDLL:
@@ -419,7 +419,8 @@ class Batch:
         out = (_BatchResult * (self.count * self.path_len))()
         _check(lib.acdc_batch_results(self._h, out, self.count * self.path_len))
         dt = np.dtype([("lambda", "f8"), ("primal", "f8"), ("dual", "f8"), ("rel_gap", "f8"),
-                       ("nnz_x", "f8"), ("epochs", "i4"), ("converged", "i4"), ("updates", "u8")])
+                       ("nnz_x", "f8"), ("epochs", "i4"), ("converged", "i4"), ("updates", "u8"),
+                       ("device_us", "f8")])
         return np.frombuffer(bytes(out), dtype=dt).reshape(self.count, self.path_len).copy()
 
     def x(self, i: int) -> np.ndarray:

@@ -59,13 +59,14 @@ struct acdc_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   BatchInstance *dinst = nullptr;
   CoordRec *dcs = nullptr;    // records of all instances, contiguous
-  int32_t *dorder = nullptr;  // [lds list | global list]
-  int32_t nlds = 0, nglob = 0;
+  int32_t *dorder = nullptr;  // instances in processing order (estimated cost, largest first)
   double *dlambda = nullptr;
   unsigned int *dqueue = nullptr;
   BatchResult *dres = nullptr;
-  int blocks_lds = 0, blocks_glob = 0;
+  int blocks = 0;  // resident blocks of the batch kernel
+  int64_t maxlen = 0;  // longest column of the batch
   double last_ms = 0;
+  double clock_khz = 1e5;  // device wall clock (hipDeviceAttributeWallClockRate)
   std::vector<void *> allocs;
 };
 
@@ -95,59 +96,64 @@ void destroy_batch(acdc_batch *b) {
 // column entries cached in registers per lane: 4 chunks, 2 for 64-lane groups
 constexpr int regs_for(int lanes) { return lanes == 64 ? 2 : 4; }
 
-template <typename T, int G, bool LDSR>
-const void *batch_fn(bool async) {
-  if (async) return reinterpret_cast<const void *>(&k_batch_async<T, G, regs_for(G), LDSR>);
-  return reinterpret_cast<const void *>(&k_batch_lasso<T, G, kSlotsPerGroup, regs_for(G), LDSR>);
+// entries of a column an asynchronous worker caches in registers per lane
+// (FP32: 8 chunks for 8 and 16 lanes, 4 for 32, 2 for 64; FP64: at most 4),
+// the most that stays spill-free in 128 VGPRs; longer columns take the
+// variant with a tail loop
+template <typename T>
+constexpr int async_regs_for(int lanes) {
+  return lanes == 64 ? 2 : (lanes == 32 || sizeof(T) == 8 ? 4 : 8);
 }
 
-template <typename T, bool LDSR>
-const void *batch_fn_lanes(int lanes, bool async) {
+template <typename T, int G>
+const void *batch_fn(bool async, bool tail) {
+  constexpr int RA = async_regs_for<T>(G);
+  if (async)
+    return tail ? reinterpret_cast<const void *>(&k_batch_async<T, G, RA, true>)
+                : reinterpret_cast<const void *>(&k_batch_async<T, G, RA, false>);
+  return reinterpret_cast<const void *>(&k_batch_lasso<T, G, kSlotsPerGroup, regs_for(G)>);
+}
+
+template <typename T>
+const void *batch_fn_lanes(int lanes, bool async, bool tail) {
   switch (lanes) {
-    case 8: return batch_fn<T, 8, LDSR>(async);
-    case 16: return batch_fn<T, 16, LDSR>(async);
-    case 32: return batch_fn<T, 32, LDSR>(async);
-    default: return batch_fn<T, 64, LDSR>(async);
+    case 8: return batch_fn<T, 8>(async, tail);
+    case 16: return batch_fn<T, 16>(async, tail);
+    case 32: return batch_fn<T, 32>(async, tail);
+    default: return batch_fn<T, 64>(async, tail);
   }
 }
 
-const void *batch_kernel(const acdc_batch *b, bool lds) {
-  if (b->fp32)
-    return lds ? batch_fn_lanes<float, true>(b->lanes, b->async)
-               : batch_fn_lanes<float, false>(b->lanes, b->async);
-  return lds ? batch_fn_lanes<double, true>(b->lanes, b->async)
-             : batch_fn_lanes<double, false>(b->lanes, b->async);
+const void *batch_kernel(const acdc_batch *b) {
+  const int ra = b->fp32 ? async_regs_for<float>(b->lanes) : async_regs_for<double>(b->lanes);
+  const bool tail = b->maxlen > (int64_t)ra * b->lanes;
+  return b->fp32 ? batch_fn_lanes<float>(b->lanes, b->async, tail)
+                 : batch_fn_lanes<double>(b->lanes, b->async, tail);
 }
 
-// enqueue reset + the two persistent launches on b->stream
+// enqueue the reset and the persistent launch on b->stream
 acdc_status enqueue_run(acdc_batch *b) {
-  const int64_t nreset = std::max<int64_t>(b->total_n, 2);
+  const int64_t nreset = std::max<int64_t>(b->total_n, 1);
   k_batch_reset<<<(unsigned)((nreset + kBlock - 1) / kBlock), kBlock, 0, b->stream>>>(
-      b->dcs, b->total_n, b->dqueue, 2);
+      b->dcs, b->total_n, b->dqueue, 1);
   BHIP(hipGetLastError());
   const size_t tsz = b->fp32 ? 4 : 8;
-  for (int pass = 0; pass < 2; ++pass) {
-    const bool lds = pass == 0;
-    const int32_t cnt = lds ? b->nlds : b->nglob;
-    if (cnt == 0) continue;
-    BatchArgs a;
-    a.inst = b->dinst;
-    a.order = b->dorder + (lds ? 0 : b->nlds);
-    a.count = cnt;
-    a.path_len = b->path_len;
-    a.lambdas = b->dlambda;
-    a.refresh_period = b->set.refresh_period;
-    a.max_epochs = b->set.max_epochs;
-    a.gap_tol = b->set.gap_tol;
-    a.queue = b->dqueue + pass;
-    a.results = b->dres;
-    a.lds_doubles = lds ? (int32_t)(kLdsBytes / 8) : 0;
-    void *args[] = {&a};
-    const size_t shmem = lds ? (kLdsBytes / tsz) * tsz : 0;
-    const int blocks = lds ? b->blocks_lds : b->blocks_glob;
-    BHIP(hipLaunchKernel(batch_kernel(b, lds), dim3(std::min(blocks, cnt)), dim3(kBatchBlock), args,
-                         shmem, b->stream));
-  }
+  BatchArgs a;
+  a.inst = b->dinst;
+  a.order = b->dorder;
+  a.count = b->count;
+  a.path_len = b->path_len;
+  a.lambdas = b->dlambda;
+  a.refresh_period = b->set.refresh_period;
+  a.max_epochs = b->set.max_epochs;
+  a.gap_tol = b->set.gap_tol;
+  a.queue = b->dqueue;
+  a.results = b->dres;
+  a.lds_doubles = (int32_t)(kLdsBytes / 8);
+  void *args[] = {&a};
+  const size_t shmem = (kLdsBytes / tsz) * tsz;
+  BHIP(hipLaunchKernel(batch_kernel(b), dim3(std::min(b->blocks, b->count)), dim3(kBatchBlock),
+                       args, shmem, b->stream));
   return ACDC_OK;
 }
 
@@ -203,6 +209,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   b->path_len = path_len;
   int64_t tot_nnz = 0, tot_n = 0, tot_m = 0;
   for (int32_t i = 0; i < count; ++i) {
+    const std::vector<int32_t> &cp = problems[i]->cptr;
+    for (size_t c = 0; c + 1 < cp.size(); ++c) b->maxlen = std::max<int64_t>(b->maxlen, cp[c + 1] - cp[c]);
     tot_nnz += problems[i]->nnz;
     tot_n += problems[i]->ncoord;
     tot_m += problems[i]->nres;
@@ -235,18 +243,21 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   double *dval64, *db, *dr64;
   acdc_status st;
   if ((st = balloc(b, &b->dcs, tot_n)) != ACDC_OK) return bail(st);
-  if ((st = balloc(b, &dent, tot_nnz * esz)) != ACDC_OK) return bail(st);
+  // one pad entry after the last segment: workers load a padding lane's
+  // entry unconditionally (from the segment start, which for an empty last
+  // segment is the pad)
+  if ((st = balloc(b, &dent, (tot_nnz + 1) * esz)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &dval64, tot_nnz)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &db, tot_m)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &dr64, tot_m)) != ACDC_OK) return bail(st);
   int64_t glob_m = 0;
   for (int32_t i = 0; i < count; ++i)
-    if (problems[i]->nres > lds_rows) glob_m += problems[i]->nres;
+    if (problems[i]->nres > lds_rows) glob_m += problems[i]->nres - lds_rows;
   if (glob_m > 0 && (st = balloc(b, &ddr, glob_m * tsz)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &b->dinst, count)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &b->dorder, count)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &b->dlambda, (size_t)count * path_len)) != ACDC_OK) return bail(st);
-  if ((st = balloc(b, &b->dqueue, 2)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &b->dqueue, 1)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &b->dres, (size_t)count * path_len)) != ACDC_OK) return bail(st);
 
   std::vector<BatchInstance> hinst(count);
@@ -263,15 +274,16 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     I.val64 = dval64 + onnz;
     I.b = db + om;
     I.r64 = dr64 + om;
+    // residual rows beyond the LDS capacity go to the instance's global part
     I.drg = nullptr;
+    I.lds_rows = (int32_t)std::min<int64_t>(p->nres, lds_rows);
     if (p->nres > lds_rows) {
       I.drg = ddr + og * tsz;
-      og += p->nres;
+      og += p->nres - lds_rows;
     }
     I.m = (int32_t)p->nres;
     I.n = (int32_t)p->ncoord;
     I.tau = (int32_t)std::min<int64_t>(slots, p->ncoord);
-    I.pad = 0;
     I.seed = set->seed + (uint64_t)i;
     const double beta = p->ncoord > 1 ? 1.0 + (double)(p->omega - 1) * (double)(I.tau - 1) /
                                                   ((double)p->ncoord - 1.0)
@@ -313,15 +325,15 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   if (e != hipSuccess)
     return bail(bfail(ACDC_ERR_HIP, std::string("acdc_batch_create: upload: ") + hipGetErrorString(e)));
 
-  // work lists: LDS-resident rows first, each list largest (nnz) first
+  // one work list, largest estimated cost first (longest-processing-time
+  // order): an instance whose residual spills past the LDS costs more per
+  // nnz than one held entirely in LDS
   std::vector<int32_t> ord(count);
   std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) {
-    const bool la = problems[a]->nres <= lds_rows, lc = problems[c]->nres <= lds_rows;
-    if (la != lc) return la;
-    return problems[a]->nnz > problems[c]->nnz;
-  });
-  for (int32_t i : ord) (problems[i]->nres <= lds_rows ? b->nlds : b->nglob)++;
+  auto cost = [&](int32_t i) {
+    return (double)problems[i]->nnz * (problems[i]->nres <= lds_rows ? 1.0 : 1.3);
+  };
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) { return cost(a) > cost(c); });
   BHIP(hipMemcpy(b->dorder, ord.data(), count * 4, hipMemcpyHostToDevice));
   BHIP(hipMemcpy(b->dinst, hinst.data(), count * sizeof(BatchInstance), hipMemcpyHostToDevice));
   BHIP(hipMemcpy(b->dlambda, lambdas, (size_t)count * path_len * 8, hipMemcpyHostToDevice));
@@ -331,15 +343,17 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   hipDeviceProp_t prop;
   BHIP(hipGetDevice(&dev));
   BHIP(hipGetDeviceProperties(&prop, dev));
-  for (int pass = 0; pass < 2; ++pass) {
-    const bool lds = pass == 0;
-    const size_t shmem = lds ? (kLdsBytes / tsz) * tsz : 0;
-    BHIP(hipFuncSetAttribute(batch_kernel(b, lds), hipFuncAttributeMaxDynamicSharedMemorySize,
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+    b->clock_khz = khz;
+  {
+    const size_t shmem = (kLdsBytes / tsz) * tsz;
+    BHIP(hipFuncSetAttribute(batch_kernel(b), hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)shmem));
     int per_cu = 0;
-    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(b, lds), kBatchBlock,
+    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(b), kBatchBlock,
                                                       shmem));
-    (lds ? b->blocks_lds : b->blocks_glob) = std::max(1, per_cu) * prop.multiProcessorCount;
+    b->blocks = std::max(1, per_cu) * prop.multiProcessorCount;
   }
   BHIP(hipStreamSynchronize(b->stream));
 
@@ -392,6 +406,7 @@ extern "C" acdc_status acdc_batch_results(acdc_batch *b, acdc_batch_result *out,
     out[k].epochs = r[k].epochs;
     out[k].converged = r[k].converged;
     out[k].updates = r[k].updates;
+    out[k].device_us = (double)r[k].ticks * 1e3 / b->clock_khz;
   }
   return ACDC_OK;
 }

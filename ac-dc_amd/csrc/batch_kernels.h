@@ -45,10 +45,10 @@ struct BatchInstance {
   const double *val64;   // FP64 values of the refresh, entry order
   const double *b;
   double *r64;           // FP64 residual of the refresh when it does not fit in LDS
-  void *drg;             // global correction residual (T) when the rows do not fit in LDS
+  void *drg;             // rows [lds_rows, m) of the correction residual (T), or null
   int32_t m, n;
   int32_t tau;           // coordinates per step of this instance
-  int32_t pad;
+  int32_t lds_rows;      // rows [0, lds_rows) of the correction residual live in LDS
   uint64_t seed;
 };
 
@@ -56,6 +56,7 @@ struct BatchResult {
   double lambda, primal, dual, rel_gap, nnz_x;
   int32_t epochs, converged;
   uint64_t updates;
+  uint64_t ticks;  // wall-clock ticks of this lambda
 };
 
 struct BatchArgs {
@@ -124,7 +125,7 @@ struct ColCache {
     for (int r = 0; r < R; ++r) {
       const int32_t q = lane + r * G;
       if (q < len) {
-        const Entry<T> e = ent[p0 + q];
+        const Entry<T> e = gld(ent + p0 + q);
         row[r] = e.row;
         val[r] = e.val;
       } else {
@@ -133,104 +134,155 @@ struct ColCache {
       }
     }
   }
-  // <column, v>, partial per lane (entries past R*G read straight from memory)
-  template <typename V>
-  __device__ __forceinline__ V dot(const Entry<T> *ent, const V *v, int lane) const {
-    V acc = 0;
+  // <column, dr>, partial per lane (entries past R*G read straight from memory)
+  template <typename View>
+  __device__ __forceinline__ T dot(const Entry<T> *ent, const View &dr, int lane) const {
+    T acc = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r)  // padding lanes read nothing
-      if (lane + r * G < len) acc += (V)val[r] * v[row[r]];
+      if (lane + r * G < len) acc += val[r] * dr.get(row[r]);
     for (int32_t q = lane + R * G; q < len; q += G) {
-      const Entry<T> e = ent[p0 + q];
-      acc += (V)e.val * v[e.row];
+      const Entry<T> e = gld(ent + p0 + q);
+      acc += e.val * dr.get(e.row);
     }
     return acc;
   }
 };
 
-template <typename T, bool LDSR>
-__device__ __forceinline__ T *dr_ptr(const BatchInstance &I, T *lds) {
-  return LDSR ? lds : static_cast<T *>(I.drg);
-}
-
-template <typename V, bool LOCAL>
-__device__ __forceinline__ void add_to(V *p, V v) {
-  if (LOCAL)
-    atomicAdd(p, v);  // ds_add on LDS
-  else
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// The correction residual of an instance: rows [0, lds_rows) in LDS, rows
+// [lds_rows, m) in the instance's global buffer (the rows of the widest
+// instances exceed the 159 KB of LDS by a few thousand).  FULL: every row in
+// LDS.  The global part is only touched by the owning block: workgroup-scope
+// atomics, agent-scope loads (not through the L1, which does not see the
+// atomics performed in L2).
+template <typename T, bool FULL>
+struct DrView {
+  T *lds;
+  T *glob;
+  int32_t split;
+  __device__ __forceinline__ T get(int32_t row) const {
+    if (FULL || row < split) return lds[row];
+    return gld_agent(&glob[row - split]);
+  }
+  __device__ __forceinline__ void add(int32_t row, T v) const {
+    if (FULL || row < split)
+      atomicAdd(&lds[row], v);  // ds_add on LDS
+    else
+      gadd_block(&glob[row - split], v);
+  }
+  __device__ __forceinline__ void zero(int32_t m) const {
+    for (int32_t k = threadIdx.x; k < m; k += blockDim.x) {
+      if (FULL || k < split)
+        lds[k] = (T)0;
+      else
+        *gp(&glob[k - split]) = (T)0;
+    }
+  }
+};
+template <typename T, bool FULL>
+__device__ __forceinline__ DrView<T, FULL> dr_view(const BatchInstance &I, T *lds) {
+  return DrView<T, FULL>{lds, static_cast<T *>(I.drg), FULL ? I.m : I.lds_rows};
 }
 
 // FP64 refresh + evaluation of one instance by one block: fills out (in
 // shared memory) and zeroes dr.  r64 lives in LDS (reusing the space of dr,
 // which is rebuilt at the end) when m doubles fit, in the global r64 buffer
-// otherwise.  The gradient pass walks the columns in rounds of
-// kBatchBlock/16 x 2 columns, one 16-lane group per column.  Its statistics
-// meet in thread 0 only (block_stats), which keeps the pass near 36 VGPRs so
-// that inlining it costs the steps nothing.
+// otherwise.  The scatter of the nonzero x_j gives each thread its own
+// columns; the gradient pass gives each 16-lane group two columns per round
+// with all their loads in flight together.  Statistics meet in thread 0 only
+// (block_stats).
 template <typename T, bool LDSR>
 __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double lambda, T *lds,
                                            double *sh, int lds_doubles, BatchResult *out) {
-  constexpr int G = 16, NG = kBatchBlock / G;
   const BatchInstance &I = *Ip;
-  const int tid = threadIdx.x, lane = tid & (G - 1), grp = tid / G;
+  const int tid = threadIdx.x;
   const bool rl = LDSR && I.m <= lds_doubles;  // block-uniform
-  double *r = rl ? reinterpret_cast<double *>(lds) : I.r64;
+  double *rs = reinterpret_cast<double *>(lds);  // r64 in LDS when rl
+  double *rg = I.r64;                            // r64 in global memory otherwise
+  // a global r64 takes atomics in L2, so it is read past the L1
+  auto rget = [&](int32_t row) -> double { return rl ? rs[row] : gld_agent(rg + row); };
+  auto radd = [&](int32_t row, double v) {
+    if (rl)
+      atomicAdd(rs + row, v);
+    else
+      gadd_block(rg + row, v);
+  };
   const Entry<T> *ent = static_cast<const Entry<T> *>(I.ent);
-  for (int k = tid; k < I.m; k += kBatchBlock) r[k] = -I.b[k];
-  __syncthreads();
-  // r += x_j a_j over the nonzero x_j: a 16-lane group per column
-  for (int j = grp; j < I.n; j += NG) {
-    const double xj = ld_coherent(&I.cs[j].x);  // async workers add to x atomically
-    if (xj == 0.0) continue;
-    const int32_t p0 = I.cs[j].p0, len = I.cs[j].len;
-    for (int32_t q = lane; q < len; q += G) {
-      const int32_t p = p0 + q;
-      const double v = I.val64[p] * xj;
-      if (rl)
-        add_to<double, true>(&r[ent[p].row], v);
-      else
-        add_to<double, false>(&r[ent[p].row], v);
-    }
+  for (int k = tid; k < I.m; k += kBatchBlock) {
+    const double v = -gld(I.b + k);
+    if (rl)
+      rs[k] = v;
+    else
+      *gp(rg + k) = v;
   }
   __syncthreads();
+  // r += x_j a_j over the nonzero x_j
+  for (int j = tid; j < I.n; j += kBatchBlock) {
+    const double xj = gld_agent(&I.cs[j].x);  // async workers add to x atomically
+    if (xj == 0.0) continue;
+    const int32_t p0 = gld(&I.cs[j].p0), len = gld(&I.cs[j].len);
+#pragma unroll 4
+    for (int32_t q = 0; q < len; ++q) radd(gld(ent + p0 + q).row, gld(I.val64 + p0 + q) * xj);
+  }
+  __syncthreads();
+  // g_j = <a_j, r64>: a 16-lane group per column, two columns per group and
+  // round, every load of the round (records, then up to 64 entries and
+  // values per column) issued before the first use
+  constexpr int G = 16, NG = kBatchBlock / G, S = 2, RC = 4;
+  const int lane = tid & (G - 1), grp = tid / G;
   double ginf = 0, x1 = 0, nz = 0;
-  // two columns per group and round, both loads issued before the reductions
-  for (int j0 = grp; j0 < I.n; j0 += 2 * NG) {
-    const int j1 = j0 + NG;
-    const bool on1 = j1 < I.n;
-    const int32_t pa = I.cs[j0].p0, la = I.cs[j0].len;
-    const int32_t pb = on1 ? I.cs[j1].p0 : 0, lb = on1 ? I.cs[j1].len : 0;
-    double ga = 0, gb = 0;
-    for (int32_t q = lane; q < la; q += G) ga += I.val64[pa + q] * r[ent[pa + q].row];
-    for (int32_t q = lane; q < lb; q += G) gb += I.val64[pb + q] * r[ent[pb + q].row];
-    ga = group_sum<G>(ga);
-    gb = group_sum<G>(gb);
-    if (lane == 0) {
-      const double xa = ld_coherent(&I.cs[j0].x);
-      I.cs[j0].gref = ga;
-      ginf = fmax(ginf, fabs(ga));
-      x1 += fabs(xa);
-      nz += xa != 0.0;
-      if (on1) {
-        const double xb = ld_coherent(&I.cs[j1].x);
-        I.cs[j1].gref = gb;
-        ginf = fmax(ginf, fabs(gb));
-        x1 += fabs(xb);
-        nz += xb != 0.0;
-      }
+  for (int j0 = grp; j0 < I.n; j0 += S * NG) {
+    int32_t p0[S], len[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int j = min(j0 + k * NG, I.n - 1);  // past the end: a repeat, dropped below
+      p0[k] = gld(&I.cs[j].p0);
+      len[k] = gld(&I.cs[j].len);
     }
+    int32_t row[S][RC];
+    double val[S][RC];
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+#pragma unroll
+      for (int r = 0; r < RC; ++r) {
+        const int32_t q = lane + r * G;
+        const bool in = q < len[k];
+        const int32_t p = p0[k] + (in ? q : 0);
+        row[k][r] = in ? gld(ent + p).row : 0;
+        val[k][r] = in ? gld(I.val64 + p) : 0.0;
+      }
+    double g[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      g[k] = 0;
+#pragma unroll
+      for (int r = 0; r < RC; ++r) g[k] += val[k][r] * rget(row[k][r]);
+      for (int32_t q = lane + RC * G; q < len[k]; q += G)
+        g[k] += gld(I.val64 + p0[k] + q) * rget(gld(ent + p0[k] + q).row);
+      g[k] = group_sum<G>(g[k]);
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        const int j = j0 + k * NG;
+        if (j < I.n) {
+          *gp(&I.cs[j].gref) = g[k];
+          const double x = gld_agent(&I.cs[j].x);
+          ginf = fmax(ginf, fabs(g[k]));
+          x1 += fabs(x);
+          nz += x != 0.0;
+        }
+      }
   }
   double r2 = 0, br = 0;
   for (int k = tid; k < I.m; k += kBatchBlock) {
-    const double rk = r[k];
+    const double rk = rget(k);
     r2 += rk * rk;
-    br += I.b[k] * rk;
+    br += gld(I.b + k) * rk;
   }
   double st[kRefreshStats] = {ginf, x1, nz, r2, br};
   block_stats(st, sh);  // its barrier also ends every read of r
-  T *dr = dr_ptr<T, LDSR>(I, lds);
-  for (int k = tid; k < I.m; k += kBatchBlock) dr[k] = (T)0;
+  dr_view<T, LDSR>(I, lds).zero(I.m);
   if (tid == 0) {
     const double P = 0.5 * st[3] + lambda * st[1];
     const double s = st[0] > lambda ? lambda / st[0] : 1.0;
@@ -259,7 +311,7 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
   const int wgrp0 = (threadIdx.x >> 6) * GW;  // first group of this wave
   const Perm perm = Perm::make((uint32_t)I.n);
   const Entry<T> *ent = static_cast<const Entry<T> *>(I.ent);
-  T *dr = dr_ptr<T, LDSR>(I, lds);
+  const DrView<T, LDSR> dr = dr_view<T, LDSR>(I, lds);
   StepScalars sc;
   sc.lambda = lambda;
   sc.lambda_n = 0;
@@ -279,7 +331,7 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
     for (int k = 0; k < S; ++k) {
       j[k] = (uint32_t)__shfl((int)mine, k * GW + (grp - wgrp0), 64);
       if (grp + k * NG < I.tau) {
-        rec[k] = I.cs[j[k]];
+        rec[k] = gld(I.cs + j[k]);
       } else {
         rec[k].x = rec[k].L = rec[k].gref = 0.0;
         rec[k].p0 = rec[k].len = 0;
@@ -299,7 +351,7 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
       acc = group_sum<G>(acc);
       const double g = rec[k].gref + (double)acc;
       const double d = prox_delta<kLasso>(g, rec[k].x, rec[k].L, 0.0, sc, &c[k]);
-      if (lane == 0) I.cs[j[k]].x = rec[k].x + d;
+      if (lane == 0) *gp(&I.cs[j[k]].x) = rec[k].x + d;
     }
     __syncthreads();
 #pragma unroll
@@ -308,10 +360,10 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
       const T ct = (T)c[k];
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        if (lane + r * G < col[k].len) add_to<T, LDSR>(&dr[col[k].row[r]], ct * col[k].val[r]);
+        if (lane + r * G < col[k].len) dr.add(col[k].row[r], ct * col[k].val[r]);
       for (int32_t q = lane + R * G; q < col[k].len; q += G) {
-        const Entry<T> e = ent[col[k].p0 + q];
-        add_to<T, LDSR>(&dr[e.row], ct * e.val);
+        const Entry<T> e = gld(ent + col[k].p0 + q);
+        dr.add(e.row, ct * e.val);
       }
     }
     if (s + 1 < nsteps) prefetch(step0 + s + 1);
@@ -319,7 +371,10 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
   }
 }
 
-template <typename T, int G, int S, int R, bool LDSR>
+// An instance's correction residual lives in LDS when the host gave it no
+// global buffer (drg == nullptr); each block branches on that per instance,
+// so one launch and one queue serve both placements.
+template <typename T, int G, int S, int R>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
@@ -337,29 +392,45 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
     const int i = a.order[item];
     const BatchInstance *Ip = a.inst + i;
     const BatchInstance I = *Ip;
+    const bool in_lds = I.drg == nullptr;
+    auto refresh = [&](double lambda) {
+      if (in_lds)
+        batch_refresh<T, true>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+      else
+        batch_refresh<T, false>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+    };
     const uint64_t steps_per_epoch = ((uint64_t)I.n + I.tau - 1) / I.tau;
     uint64_t step = 0;
     for (int l = 0; l < a.path_len; ++l) {
+      const uint64_t t0 = wall_clock64();
       const double lambda = a.lambdas[(int64_t)i * a.path_len + l];
-      batch_refresh<T, LDSR>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+      refresh(lambda);
       double gap = s_res.rel_gap;
       int epochs = 0;
       while (gap > a.gap_tol && epochs < a.max_epochs) {
         const int ep = min(a.refresh_period, a.max_epochs - epochs);
         const uint64_t ns = steps_per_epoch * (uint64_t)ep;
-        batch_steps<T, G, S, R, LDSR>(I, lambda, step, ns, lds);
+        if (in_lds)
+          batch_steps<T, G, S, R, true>(I, lambda, step, ns, lds);
+        else
+          batch_steps<T, G, S, R, false>(I, lambda, step, ns, lds);
         step += ns;
         epochs += ep;
-        batch_refresh<T, LDSR>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+        refresh(lambda);
         gap = s_res.rel_gap;
       }
       if (threadIdx.x == 0) {
-        BatchResult res = s_res;
-        res.lambda = lambda;
-        res.epochs = epochs;
-        res.converged = gap <= a.gap_tol;
-        res.updates = step * (uint64_t)I.tau;
-        a.results[(int64_t)i * a.path_len + l] = res;
+        const uint64_t ticks = wall_clock64() - t0;
+        BatchResult *o = a.results + (int64_t)i * a.path_len + l;
+        o->lambda = lambda;
+        o->primal = s_res.primal;
+        o->dual = s_res.dual;
+        o->rel_gap = s_res.rel_gap;
+        o->nnz_x = s_res.nnz_x;
+        o->epochs = epochs;
+        o->converged = gap <= a.gap_tol;
+        o->updates = step * (uint64_t)I.tau;
+        o->ticks = ticks;
       }
       __syncthreads();  // s_res is rewritten by the next refresh
     }
@@ -373,77 +444,96 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
 // the instance seed and the group) and apply updates with no barrier between
 // them -- LDS atomics on the residual correction, atomic adds on x_j.  Only
 // the FP64 refresh, every refresh_period epochs, stops all workers (one
-// block barrier).  An update reads its 32-byte record two updates ahead and
-// its entries one update ahead, so its critical path is the LDS gather plus
-// the agent-scope load of x_j (agent scope: the L1 does not hold x_j across
-// the other workers' atomic updates).
-template <typename T, int G, int R, bool LDSR>
+// block barrier).  An update reads its 32-byte record (x_j included) two
+// updates ahead and its entries one update ahead, so its critical path is
+// the LDS gather: three or four memory requests per update, the resource a
+// CU runs out of first.
+template <typename T, int G, int R, bool LDSR, bool TAIL>
 __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, double lambda,
                                                     uint64_t k0, uint64_t count, T *lds) {
   const int lane = threadIdx.x & (G - 1);
   const uint32_t w = threadIdx.x / G;
   if ((int32_t)w >= I.tau || count == 0) return;
   const Entry<T> *ent = static_cast<const Entry<T> *>(I.ent);
-  T *dr = dr_ptr<T, LDSR>(I, lds);
+  const DrView<T, LDSR> dr = dr_view<T, LDSR>(I, lds);
   StepScalars sc;
   sc.lambda = lambda;
   sc.lambda_n = 0;
   sc.inv_lambda_n = 0;
   const bool serial = I.tau == 1;
+  // this worker's last two updates (coordinate, delta)
+  uint32_t j1 = 0xffffffffu, j2 = 0xffffffffu;
+  double d1 = 0.0, d2 = 0.0;
   struct Slot {
     uint32_t j;
     CoordRec r;
     int32_t ci[R];
     T cv[R];
   };
+  AsyncCoordCache<G> coords;
   auto record = [&](uint64_t k, Slot &c) {
-    c.j = async_coord(I.seed, w, k0 + k, (uint32_t)I.n);
-    c.r = I.cs[c.j];
+    c.j = coords.get(I.seed, w, k0 + k, (uint32_t)I.n);
+    c.r = gld(I.cs + c.j);
   };
+  // every lane loads (padding lanes re-read the segment's first entry, or
+  // the pad entry after the last segment), so that the number of memory
+  // instructions in flight is the same on every path and the wait before
+  // an update's gathers only covers loads issued one update earlier
   auto entries = [&](Slot &c) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int32_t q = lane + r * G;
-      if (q < c.r.len) {
-        const Entry<T> e = ent[c.r.p0 + q];
-        c.ci[r] = e.row;
-        c.cv[r] = e.val;
-      } else {
-        c.ci[r] = 0;
-        c.cv[r] = (T)0;
-      }
+      const bool in = q < c.r.len;
+      const Entry<T> e = gld(ent + c.r.p0 + (in ? q : 0));
+      c.ci[r] = in ? e.row : 0;
+      c.cv[r] = in ? e.val : (T)0;
     }
   };
-  auto gather = [&](int32_t row) -> T {
-    return LDSR ? dr[row] : ld_coherent(&dr[row]);
-  };
+  auto gather = [&](int32_t row) -> T { return dr.get(row); };
   auto update = [&](uint64_t k, Slot &cur, Slot &nxt, Slot &ahead) {
-    if (k + 1 < count) entries(nxt);
-    if (k + 2 < count) record(k + 2, ahead);
-    const double xj = ld_coherent(&I.cs[cur.j].x);
+    entries(nxt);        // update k+1 (past the last update: harmless loads)
+    record(k + 2, ahead);
+    // x_j from the record loaded two updates ago, plus this worker's own
+    // changes to it since (the two previous updates, applied in their order,
+    // which makes one worker the serial method exactly); other workers'
+    // changes in between are the staleness asynchronous updates allow, and
+    // the atomic add below keeps them
+    double xj = cur.r.x;
+    if (cur.j == j2) xj += d2;
+    if (cur.j == j1) xj += d1;
+    // gathers of every lane, without branches (padding lanes read row 0 and
+    // weigh it by 0), all issued before any is used
+    T v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = gather(cur.ci[r]);
     T acc = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (lane + r * G < cur.r.len) acc += cur.cv[r] * gather(cur.ci[r]);
-    for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
-      const Entry<T> e = ent[cur.r.p0 + q];
-      acc += e.val * gather(e.row);
-    }
+    for (int r = 0; r < R; ++r) acc += cur.cv[r] * v[r];
+    if (TAIL)
+      for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
+        const Entry<T> e = gld(ent + cur.r.p0 + q);
+        acc += e.val * gather(e.row);
+      }
     acc = group_sum<G>(acc);
     double c;
     const double d = prox_delta<kLasso>(cur.r.gref + (double)acc, xj, cur.r.L, 0.0, sc, &c);
-    if (lane == 0 && d != 0.0) unsafeAtomicAdd(&I.cs[cur.j].x, d);
+    if (lane == 0 && d != 0.0) gadd_agent(&I.cs[cur.j].x, d);
+    j2 = j1;
+    d2 = d1;
+    j1 = cur.j;
+    d1 = d;
     if (c != 0.0) {
       const T ct = (T)c;
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        if (lane + r * G < cur.r.len) add_to<T, LDSR>(&dr[cur.ci[r]], ct * cur.cv[r]);
-      for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
-        const Entry<T> e = ent[cur.r.p0 + q];
-        add_to<T, LDSR>(&dr[e.row], ct * e.val);
-      }
+        if (lane + r * G < cur.r.len) dr.add(cur.ci[r], ct * cur.cv[r]);
+      if (TAIL)
+        for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
+          const Entry<T> e = gld(ent + cur.r.p0 + q);
+          dr.add(e.row, ct * e.val);
+        }
     }
-    if (serial)  // one worker: the serial method, each update sees the last
+    if (serial)  // one worker: the serial method, each gather sees the last scatter
       asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
     cur.j = ahead.j;
     cur.r = ahead.r;
@@ -451,14 +541,16 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
   Slot s0, s1, ahead;
   record(0, s0);
   entries(s0);
-  if (count > 1) record(1, s1);
-  for (uint64_t k = 0; k < count; k += 2) {
+  record(1, s1);
+  uint64_t k = 0;
+  for (; k + 1 < count; k += 2) {  // no conditional memory work inside
     update(k, s0, s1, ahead);
-    if (k + 1 < count) update(k + 1, s1, s0, ahead);
+    update(k + 1, s1, s0, ahead);
   }
+  if (k < count) update(k, s0, s1, ahead);
 }
 
-template <typename T, int G, int R, bool LDSR>
+template <typename T, int G, int R, bool TAIL>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
@@ -474,29 +566,45 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
     const int i = a.order[item];
     const BatchInstance *Ip = a.inst + i;
     const BatchInstance I = *Ip;
+    const bool in_lds = I.drg == nullptr;
+    auto refresh = [&](double lambda) {
+      if (in_lds)
+        batch_refresh<T, true>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+      else
+        batch_refresh<T, false>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+    };
     uint64_t k = 0;  // updates per worker so far
     for (int l = 0; l < a.path_len; ++l) {
+      const uint64_t t0 = wall_clock64();
       const double lambda = a.lambdas[(int64_t)i * a.path_len + l];
-      batch_refresh<T, LDSR>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+      refresh(lambda);
       double gap = s_res.rel_gap;
       int epochs = 0;
       while (gap > a.gap_tol && epochs < a.max_epochs) {
         const int ep = min(a.refresh_period, a.max_epochs - epochs);
         const uint64_t per = ((uint64_t)I.n * (uint64_t)ep + I.tau - 1) / I.tau;
-        batch_async_updates<T, G, R, LDSR>(I, lambda, k, per, lds);
+        if (in_lds)
+          batch_async_updates<T, G, R, true, TAIL>(I, lambda, k, per, lds);
+        else
+          batch_async_updates<T, G, R, false, TAIL>(I, lambda, k, per, lds);
         __syncthreads();  // every worker's updates are in before the refresh
         k += per;
         epochs += ep;
-        batch_refresh<T, LDSR>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+        refresh(lambda);
         gap = s_res.rel_gap;
       }
       if (threadIdx.x == 0) {
-        BatchResult res = s_res;
-        res.lambda = lambda;
-        res.epochs = epochs;
-        res.converged = gap <= a.gap_tol;
-        res.updates = k * (uint64_t)I.tau;
-        a.results[(int64_t)i * a.path_len + l] = res;
+        const uint64_t ticks = wall_clock64() - t0;
+        BatchResult *o = a.results + (int64_t)i * a.path_len + l;
+        o->lambda = lambda;
+        o->primal = s_res.primal;
+        o->dual = s_res.dual;
+        o->rel_gap = s_res.rel_gap;
+        o->nnz_x = s_res.nnz_x;
+        o->epochs = epochs;
+        o->converged = gap <= a.gap_tol;
+        o->updates = k * (uint64_t)I.tau;
+        o->ticks = ticks;
       }
       __syncthreads();
     }

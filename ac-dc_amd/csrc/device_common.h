@@ -73,14 +73,44 @@ ACDC_HD StepKeys step_keys(uint64_t seed, uint64_t step) {
 // word of the 64 x 32-bit product (multiply-shift; bias below n / 2^64).
 constexpr uint32_t kAsyncTag = 0x50434441u;  // "PCDA"
 
-ACDC_HD uint32_t async_coord(uint64_t seed, uint32_t worker, uint64_t k, uint32_t n) {
-  const uint64_t blk = k >> 1;
-  uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), worker, kAsyncTag};
-  Philox::gen(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const uint32_t lo = (k & 1) ? c[2] : c[0], hi = (k & 1) ? c[3] : c[1];
-  // floor((hi 2^32 + lo) n / 2^64), exact in 64-bit pieces
+// floor((hi 2^32 + lo) n / 2^64), exact in 64-bit pieces
+ACDC_HD uint32_t mulshift64(uint32_t lo, uint32_t hi, uint32_t n) {
   return (uint32_t)(((uint64_t)hi * n + (((uint64_t)lo * n) >> 32)) >> 32);
 }
+
+// coordinates of updates 2 blk and 2 blk + 1 of a worker (one Philox block)
+ACDC_HD void async_pair(uint64_t seed, uint32_t worker, uint64_t blk, uint32_t n, uint32_t &even,
+                        uint32_t &odd) {
+  uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), worker, kAsyncTag};
+  Philox::gen(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  even = mulshift64(c[0], c[1], n);
+  odd = mulshift64(c[2], c[3], n);
+}
+
+ACDC_HD uint32_t async_coord(uint64_t seed, uint32_t worker, uint64_t k, uint32_t n) {
+  uint32_t e, o;
+  async_pair(seed, worker, k >> 1, n, e, o);
+  return (k & 1) ? o : e;
+}
+
+// A G-lane group's cache of its worker's coordinates: lane l holds the
+// Philox block B + l (updates 2(B+l), 2(B+l)+1), so one generator call per
+// lane serves 2G updates, handed out by a lane shuffle.  Updates must be
+// requested in increasing order by every group of the wave together (the
+// refill branch is then uniform).
+template <int G>
+struct AsyncCoordCache {
+  uint64_t B = ~0ull >> 1;  // first request refills
+  uint32_t even = 0, odd = 0;
+  __device__ __forceinline__ uint32_t get(uint64_t seed, uint32_t worker, uint64_t u, uint32_t n) {
+    const uint64_t blk = u >> 1;
+    if (blk - B >= (uint64_t)G) {
+      B = blk & ~(uint64_t)(G - 1);
+      async_pair(seed, worker, B + (threadIdx.x & (G - 1)), n, even, odd);
+    }
+    return (uint32_t)__shfl((int)((u & 1) ? odd : even), (int)(blk - B), G);
+  }
+};
 
 // Feistel round function: a 32-bit multiply/xor-shift integer mixer.
 ACDC_HD uint32_t mix32(uint32_t x, uint32_t k) {
@@ -128,13 +158,38 @@ struct Perm {
 };
 
 // ---------------------------------------------------------------- reductions
+// DPP lane moves within rows of 16 lanes (VALU latency, no LDS crossbar):
+// quad_perm [1,0,3,2] (lane ^ 1), quad_perm [2,3,0,1] (lane ^ 2),
+// row_half_mirror (lane i <- 7 - i within 8), row_mirror (i <- 15 - i).
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // Sum over aligned groups of G lanes (G = 8..64) of one 64-wide wavefront;
-// every lane of the group receives the group total.  The xor butterfly has a
-// fixed order, so results are reproducible run to run.
+// every lane of the group receives the same total.  Inside a row of 16 the
+// partial sums combine by DPP moves -- pairs, quads, the mirrored quad, the
+// mirrored half -- so each lane adds the same two values in one of two
+// orders, and addition commutes exactly; wider groups then add across rows
+// with xor shuffles.  The order is fixed, so results are reproducible.
 template <int G, typename T>
 __device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  static_assert(G == 8 || G == 16 || G == 32 || G == 64, "groups of 8..64 lanes");
+  v += dpp_move<kDppXor1>(v);
+  v += dpp_move<kDppXor2>(v);
+  v += dpp_move<kDppHalfMirror>(v);
+  if (G >= 16) v += dpp_move<kDppMirror>(v);
+  if (G >= 32) v += __shfl_xor(v, 16, 64);
+  if (G >= 64) v += __shfl_xor(v, 32, 64);
   return v;
 }
 

@@ -31,6 +31,38 @@ struct Seg {
 
 constexpr int kBlock = 256;  // threads per block
 
+// Global-memory (address space 1) views of generic pointers.  Memory
+// operations through a generic pointer compile to flat_* instructions, which
+// count against the LDS counter as well, so every LDS access after one waits
+// for the flat load's memory round trip; through these views they are
+// global_* instructions with their own counter.
+template <typename T>
+struct GlobalOf {
+  typedef __attribute__((address_space(1))) T type;
+};
+template <typename T>
+__device__ __forceinline__ typename GlobalOf<T>::type *gp(T *p) {
+  return (typename GlobalOf<T>::type *)p;
+}
+template <typename S>
+__device__ __forceinline__ S gld(const S *p) {
+  S v;
+  __builtin_memcpy(&v, gp(p), sizeof(S));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T gld_agent(const T *p) {
+  return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void gadd_agent(T *p, T v) {
+  __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void gadd_block(T *p, T v) {
+  __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // One (index, value) pair of a segment, interleaved so that a lane's entry
 // is one 8-byte (FP32) or 16-byte (FP64) load and a segment's entries share
 // cache lines.

@@ -244,6 +244,7 @@ typedef struct acdc_batch_result {
   double lambda, primal, dual, rel_gap, nnz_x;
   int32_t epochs, converged;
   uint64_t updates; /* coordinate updates of the instance up to this lambda */
+  double device_us; /* device time of this lambda (refreshes included) */
 } acdc_batch_result;
 
 acdc_status acdc_batch_create(const acdc_problem *const *problems, int32_t count,
