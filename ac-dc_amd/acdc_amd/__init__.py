@@ -152,6 +152,10 @@ _SIGS = {
     "acdc_batch_instance_info": ([_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), _F64P,
                                   _I32P], ctypes.c_int),
     "acdc_batch_destroy": ([_P], ctypes.c_int),
+    "acdc_batch_tune": ([ctypes.POINTER(_P), ctypes.c_int32, ctypes.POINTER(BatchSettings),
+                         ctypes.POINTER(BatchSettings), _F64P], ctypes.c_int),
+    "acdc_solver_tune_lanes": ([_P, ctypes.c_int32, ctypes.POINTER(Settings), _I32P, _F64P],
+                               ctypes.c_int),
     "acdc_gen_lasso": ([ctypes.POINTER(GenLassoCfg), ctypes.POINTER(_P)], ctypes.c_int),
     "acdc_gen_svm": ([ctypes.POINTER(GenSvmCfg), ctypes.POINTER(_P)], ctypes.c_int),
     "acdc_gen_ttd": ([ctypes.c_int32, ctypes.POINTER(_P)], ctypes.c_int),
@@ -441,6 +445,30 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+def batch_tune(problems, *, seed=1, precision=PREC_FP32_REFRESH, step_mode=STEP_ASYNC,
+               refresh_period=5, max_epochs=200, gap_tol=1e-6, use_graph=True) -> dict:
+    """acdc_batch_tune: lanes per coordinate and tau for a batch."""
+    arr = (_P * len(problems))(*[p._h for p in problems])
+    base = BatchSettings(seed, precision, 0, refresh_period, max_epochs, gap_tol,
+                         int(bool(use_graph)), step_mode, 0)
+    out = BatchSettings()
+    score = ctypes.c_double()
+    _check(lib.acdc_batch_tune(arr, len(problems), ctypes.byref(base), ctypes.byref(out),
+                               ctypes.byref(score)))
+    return {"lanes_per_coord": out.lanes_per_coord, "tau": out.tau, "score": score.value}
+
+
+def solver_tune_lanes(problem, loss, *, lam, tau, seed=1, precision=PREC_FP32_REFRESH,
+                      step_mode=STEP_ASYNC) -> dict:
+    """acdc_solver_tune_lanes: the fastest lanes per coordinate for a solver."""
+    st = Settings(lam, tau, seed, BETA_OMEGA, 1.0, precision, step_mode, 0, 0, LAUNCH_AUTO)
+    lanes = ctypes.c_int32()
+    rate = ctypes.c_double()
+    _check(lib.acdc_solver_tune_lanes(problem._h, loss, ctypes.byref(st), ctypes.byref(lanes),
+                                      ctypes.byref(rate)))
+    return {"lanes_per_coord": lanes.value, "updates_per_s": rate.value}
 
 
 def lambda_path(lambda_max: float, path_len: int = 32, ratio: float = 1e-3) -> np.ndarray:

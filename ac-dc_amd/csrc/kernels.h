@@ -418,8 +418,9 @@ __global__ __launch_bounds__(kAsyncBlock) void k_async(AsyncArgs<T> a, uint64_t 
     int32_t ci[R];
     T cv[R];
   };
+  AsyncCoordCache<G> coords;  // one generator call per lane serves 2G updates
   auto record = [&](uint64_t k, Slot &c) {
-    c.j = async_coord(a.seed, (uint32_t)w, k0 + k, a.n);
+    c.j = coords.get(a.seed, (uint32_t)w, k0 + k, a.n);
     c.r = a.rec[c.j];
   };
   auto entries = [&](Slot &c) {

@@ -98,9 +98,12 @@ def batched(acdc, args):
         f"{time.perf_counter() - t:.1f}s")
     probs = [d.problem() for d in ds]
     lams = np.stack([acdc.lambda_path(d.lambda_max, args.batch_path, 1e-3) for d in ds])
+    tuned = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC)
+    log(f"[bench] batched: autotuned lanes {tuned['lanes_per_coord']}, tau {tuned['tau']}")
     kw = dict(seed=5, precision=acdc.PREC_FP32_REFRESH, refresh_period=5,
               max_epochs=args.batch_max_epochs, gap_tol=args.gap_target,
-              step_mode=acdc.STEP_ASYNC)
+              step_mode=acdc.STEP_ASYNC, lanes_per_coord=tuned["lanes_per_coord"],
+              tau=tuned["tau"])
     b = acdc.Batch(probs, lams, use_graph=True, **kw)  # uploads, captures the graph
     t0 = time.perf_counter()
     ms = b.run()
@@ -113,7 +116,7 @@ def batched(acdc, args):
            "value": upd / wall, "unit": "coord updates/s",
            "pairs_converged": float(r["converged"].mean()),
            "worst_rel_gap": float(r["rel_gap"].max()),
-           "tau_per_instance": int(b.info(0)[0])}
+           "tau_per_instance": int(b.info(0)[0]), "autotuned": tuned}
     log(f"[bench] batched: {ms:.0f} ms device, {upd:.3e} updates = {upd / wall:.3e} upd/s, "
         f"{out['pairs_converged']:.3f} of (instance, lambda) pairs converged")
     b.close()
@@ -226,7 +229,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100, help="timed epochs")
     ap.add_argument("--warmup", type=int, default=2, help="untimed epochs")
     ap.add_argument("--tau", type=int, default=4096)
-    ap.add_argument("--lanes", type=int, default=32, help="lanes per coordinate (0 = auto)")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per coordinate (0 = autotune)")
     ap.add_argument("--m", type=int, default=200_000)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--gap-target", type=float, default=1e-6)
@@ -258,11 +261,18 @@ def main():
     prob = d.problem()
     n, tau = d.n, args.tau
     mean_nnz = d.nnz / d.n
+    maxlen = int(np.diff(d.ptr).max())
 
     def make(precision, mode, launch=acdc.LAUNCH_AUTO, refresh=5, lanes=0):
         return acdc.Solver(prob, acdc.LOSS_L1_SQUARE, lam=d.lam, tau=tau, seed=2,
                            precision=precision, refresh_period=refresh, step_mode=mode,
                            launch_mode=launch, lanes_per_coord=lanes)
+
+    # ---- lanes per coordinate: the autotuner unless given
+    if args.lanes == 0:
+        tl = acdc.solver_tune_lanes(prob, acdc.LOSS_L1_SQUARE, lam=d.lam, tau=tau, seed=2)
+        args.lanes = tl["lanes_per_coord"]
+        log(f"[bench] autotuned lanes per coordinate: {args.lanes}")
 
     # ---- FP32 + FP64 refresh, asynchronous workers: the timed epochs
     s32 = make(acdc.PREC_FP32_REFRESH, acdc.STEP_ASYNC, lanes=args.lanes)
@@ -297,7 +307,8 @@ def main():
                    "step": "one epoch = n coordinate updates, from x = 0"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
-                     "kernel": "k_async<float,G,0,4,true>", "launch_ms": launch_ms,
+                     "kernel": f"k_async<float,{args.lanes},0,4,{'true' if maxlen > 4 * args.lanes else 'false'}>",
+                     "launch_ms": launch_ms,
                      "updates_per_launch": upd_per_launch,
                      "bytes_per_update": bytes_per_update, "bytes_per_launch": bytes_per_launch},
     }

@@ -260,6 +260,23 @@ acdc_status acdc_batch_instance_info(const acdc_batch *b, int32_t instance, int6
                                      double *beta, int32_t *lds_resident);
 acdc_status acdc_batch_destroy(acdc_batch *b);
 
+/* ---- autotuner (no reference counterpart: the reference fixes tau by the
+ * thread count and has no lanes-per-coordinate or queue knobs) ------------
+ * acdc_batch_tune times short runs of the batch's largest instance (4
+ * epochs at lambda = 0.1 lambda_max) for lanes per coordinate in {8,16,32}
+ * and tau in {cap/4, cap/2, cap} (cap: the block's slots or workers for
+ * those lanes) and returns *base with the lanes and tau of the best score =
+ * log(gap at x = 0 / gap after the trial) per second of device time.  The work queue hands out one instance at a
+ * time (largest first), so there is no chunk size to tune.
+ * acdc_solver_tune_lanes times two epochs (after a warm-up epoch) at lanes
+ * 16, 32 and 64 with the other settings of *base and returns the fastest. */
+acdc_status acdc_batch_tune(const acdc_problem *const *problems, int32_t count,
+                            const acdc_batch_settings *base, acdc_batch_settings *tuned,
+                            double *score);
+acdc_status acdc_solver_tune_lanes(const acdc_problem *p, int32_t loss,
+                                   const acdc_settings *base, int32_t *lanes,
+                                   double *updates_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -143,3 +143,21 @@ def test_batch_async_workers_reach_fp64_optimum(acdc, orc, small_batch, prec):
         assert e["rel_gap"] <= 1.01e-7
         assert res[i, -1]["primal"] == pytest.approx(e["primal"], rel=1e-9)
     b.close()
+
+
+def test_autotuner_picks_valid_settings(acdc, small_batch):
+    probs = [d.problem() for d in small_batch]
+    t = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC)
+    assert t["lanes_per_coord"] in (8, 16, 32)
+    assert 1 <= t["tau"] <= 1024 // t["lanes_per_coord"]
+    assert t["score"] > 0
+    lams = lambdas_of(acdc, small_batch, 2, 0.3)
+    b = acdc.Batch(probs, lams, seed=3, precision=acdc.PREC_FP32_REFRESH, max_epochs=400,
+                   gap_tol=1e-6, step_mode=acdc.STEP_ASYNC, lanes_per_coord=t["lanes_per_coord"],
+                   tau=t["tau"])
+    b.run()
+    assert b.results()["converged"].all()
+    b.close()
+    d = small_batch[1]
+    s = acdc.solver_tune_lanes(d.problem(), acdc.LOSS_L1_SQUARE, lam=d.lam, tau=256)
+    assert s["lanes_per_coord"] in (16, 32, 64) and s["updates_per_s"] > 0
