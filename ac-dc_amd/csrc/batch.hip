@@ -240,14 +240,15 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
       hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
     return bail(bfail(ACDC_ERR_HIP, "acdc_batch_create: stream/event creation failed"));
   unsigned char *dent, *ddr = nullptr;
-  double *dval64, *db, *dr64;
+  RefEntry *drent;
+  double *db, *dr64;
   acdc_status st;
   if ((st = balloc(b, &b->dcs, tot_n)) != ACDC_OK) return bail(st);
   // one pad entry after the last segment: workers load a padding lane's
   // entry unconditionally (from the segment start, which for an empty last
   // segment is the pad)
   if ((st = balloc(b, &dent, (tot_nnz + 1) * esz)) != ACDC_OK) return bail(st);
-  if ((st = balloc(b, &dval64, tot_nnz)) != ACDC_OK) return bail(st);
+  if ((st = balloc(b, &drent, tot_nnz)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &db, tot_m)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &dr64, tot_m)) != ACDC_OK) return bail(st);
   int64_t glob_m = 0;
@@ -266,12 +267,14 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   std::vector<CoordRec> recs;
   std::vector<Entry<float>> ef;
   std::vector<Entry<double>> ed;
+  std::vector<RefEntry> re;
   for (int32_t i = 0; i < count && e == hipSuccess; ++i) {
     const acdc_problem *p = problems[i];
     BatchInstance &I = hinst[i];
     I.cs = b->dcs + on;
     I.ent = dent + onnz * esz;
-    I.val64 = dval64 + onnz;
+    I.rent = drent + onnz;
+    I.nnz = (int32_t)p->nnz;
     I.b = db + om;
     I.r64 = dr64 + om;
     // residual rows beyond the LDS capacity go to the instance's global part
@@ -314,8 +317,17 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
         e = hipMemcpy(const_cast<void *>(I.ent), ed.data(), p->nnz * esz, hipMemcpyHostToDevice);
       }
     }
-    if (e == hipSuccess && p->nnz)
-      e = hipMemcpy(const_cast<double *>(I.val64), p->cval.data(), p->nnz * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && p->nnz) {
+      re.resize(p->nnz);
+      for (int64_t j = 0; j < p->ncoord; ++j)
+        for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) {
+          re[q].col = (int32_t)j;
+          re[q].row = p->cidx[q];
+          re[q].val = p->cval[q];
+        }
+      e = hipMemcpy(const_cast<RefEntry *>(I.rent), re.data(), p->nnz * sizeof(RefEntry),
+                    hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
       e = hipMemcpy(const_cast<double *>(I.b), p->rhs.data(), p->nres * 8, hipMemcpyHostToDevice);
     onnz += p->nnz;

@@ -39,16 +39,25 @@ struct alignas(32) CoordRec {
   int32_t len;  // entries in the column
 };
 
+// One entry of the refresh's FP64 copy of a column-major matrix, in entry
+// order: the column index lets a wave stream entries across column bounds.
+struct alignas(16) RefEntry {
+  int32_t col;
+  int32_t row;
+  double val;
+};
+
 struct BatchInstance {
   CoordRec *cs;          // n records
   const void *ent;       // Entry<T>[nnz]
-  const double *val64;   // FP64 values of the refresh, entry order
+  const RefEntry *rent;  // FP64 entries of the refresh, entry order
   const double *b;
   double *r64;           // FP64 residual of the refresh when it does not fit in LDS
   void *drg;             // rows [lds_rows, m) of the correction residual (T), or null
   int32_t m, n;
   int32_t tau;           // coordinates per step of this instance
   int32_t lds_rows;      // rows [0, lds_rows) of the correction residual live in LDS
+  int32_t nnz;
   uint64_t seed;
 };
 
@@ -188,15 +197,17 @@ __device__ __forceinline__ DrView<T, FULL> dr_view(const BatchInstance &I, T *ld
 // shared memory) and zeroes dr.  r64 lives in LDS (reusing the space of dr,
 // which is rebuilt at the end) when m doubles fit, in the global r64 buffer
 // otherwise.  The scatter of the nonzero x_j gives each thread its own
-// columns; the gradient pass gives each 16-lane group two columns per round
-// with all their loads in flight together.  Statistics meet in thread 0 only
-// (block_stats).
-template <typename T, bool LDSR>
-__device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double lambda, T *lds,
-                                           double *sh, int lds_doubles, BatchResult *out) {
-  const BatchInstance &I = *Ip;
+// columns; the gradient pass streams the (column, row, value) entries in
+// wave-contiguous ranges and sums columns by segmented scans.  Statistics
+// meet in thread 0 only (block_stats).
+template <typename T, bool LDSR, bool RL>
+__device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, double lambda, T *lds,
+                                                double *sh, BatchResult *out, bool dense) {
+  // a copy: through a reference every store below could alias the
+  // instance's fields, which would then be re-read after each store
+  const BatchInstance I = *Ip;
   const int tid = threadIdx.x;
-  const bool rl = LDSR && I.m <= lds_doubles;  // block-uniform
+  constexpr bool rl = RL;  // r64 in LDS
   double *rs = reinterpret_cast<double *>(lds);  // r64 in LDS when rl
   double *rg = I.r64;                            // r64 in global memory otherwise
   // a global r64 takes atomics in L2, so it is read past the L1
@@ -207,7 +218,6 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
     else
       gadd_block(rg + row, v);
   };
-  const Entry<T> *ent = static_cast<const Entry<T> *>(I.ent);
   for (int k = tid; k < I.m; k += kBatchBlock) {
     const double v = -gld(I.b + k);
     if (rl)
@@ -216,63 +226,175 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
       *gp(rg + k) = v;
   }
   __syncthreads();
-  // r += x_j a_j over the nonzero x_j
-  for (int j = tid; j < I.n; j += kBatchBlock) {
-    const double xj = gld_agent(&I.cs[j].x);  // async workers add to x atomically
-    if (xj == 0.0) continue;
-    const int32_t p0 = gld(&I.cs[j].p0), len = gld(&I.cs[j].len);
-#pragma unroll 4
-    for (int32_t q = 0; q < len; ++q) radd(gld(ent + p0 + q).row, gld(I.val64 + p0 + q) * xj);
-  }
-  __syncthreads();
-  // g_j = <a_j, r64>: a 16-lane group per column, two columns per group and
-  // round, every load of the round (records, then up to 64 entries and
-  // values per column) issued before the first use
-  constexpr int G = 16, NG = kBatchBlock / G, S = 2, RC = 4;
-  const int lane = tid & (G - 1), grp = tid / G;
+  // r += x_j a_j over the nonzero x_j.  ||x||_1 and nnz(x) are counted here.
   double ginf = 0, x1 = 0, nz = 0;
-  for (int j0 = grp; j0 < I.n; j0 += S * NG) {
-    int32_t p0[S], len[S];
+  const int wv = tid >> 6, ln = tid & 63;
+  // wave w's contiguous share of the entries (both streaming passes)
+  const int32_t e_lo = (int32_t)((int64_t)I.nnz * wv / kBatchWaves);
+  const int32_t e_hi = (int32_t)((int64_t)I.nnz * (wv + 1) / kBatchWaves);
+  constexpr int U = 4;  // steps of 64 entries per wave and round
+  if (!dense) {
+    // few nonzeros: each thread takes its own columns, the x of four columns
+    // loaded before any is tested
+    constexpr int XU = 4;
+    for (int j0 = tid; j0 < I.n; j0 += XU * kBatchBlock) {
+      double xs[XU];
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
-      const int j = min(j0 + k * NG, I.n - 1);  // past the end: a repeat, dropped below
-      p0[k] = gld(&I.cs[j].p0);
-      len[k] = gld(&I.cs[j].len);
-    }
-    int32_t row[S][RC];
-    double val[S][RC];
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-#pragma unroll
-      for (int r = 0; r < RC; ++r) {
-        const int32_t q = lane + r * G;
-        const bool in = q < len[k];
-        const int32_t p = p0[k] + (in ? q : 0);
-        row[k][r] = in ? gld(ent + p).row : 0;
-        val[k][r] = in ? gld(I.val64 + p) : 0.0;
+      for (int u = 0; u < XU; ++u) {
+        const int j = j0 + u * kBatchBlock;
+        xs[u] = j < I.n ? gld_agent(&I.cs[j].x) : 0.0;  // async workers add to x atomically
       }
-    double g[S];
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
-      g[k] = 0;
-#pragma unroll
-      for (int r = 0; r < RC; ++r) g[k] += val[k][r] * rget(row[k][r]);
-      for (int32_t q = lane + RC * G; q < len[k]; q += G)
-        g[k] += gld(I.val64 + p0[k] + q) * rget(gld(ent + p0[k] + q).row);
-      g[k] = group_sum<G>(g[k]);
-    }
-    if (lane == 0)
-#pragma unroll
-      for (int k = 0; k < S; ++k) {
-        const int j = j0 + k * NG;
-        if (j < I.n) {
-          *gp(&I.cs[j].gref) = g[k];
-          const double x = gld_agent(&I.cs[j].x);
-          ginf = fmax(ginf, fabs(g[k]));
-          x1 += fabs(x);
-          nz += x != 0.0;
+      for (int u = 0; u < XU; ++u) {
+        if (xs[u] == 0.0) continue;
+        x1 += fabs(xs[u]);
+        nz += 1.0;
+        const int j = j0 + u * kBatchBlock;
+        const int32_t p0 = gld(&I.cs[j].p0), len = gld(&I.cs[j].len);
+#pragma unroll 4
+        for (int32_t q = 0; q < len; ++q) {
+          const RefEntry e = gld(I.rent + p0 + q);
+          radd(e.row, e.val * xs[u]);
         }
       }
+    }
+  } else {
+    // many nonzeros: the statistics over the records, then every entry
+    // streamed (wave-contiguous ranges, the next round's loads in flight)
+    // with its column's x gathered beside it
+    constexpr int XU = 8;
+    for (int j0 = tid; j0 < I.n; j0 += XU * kBatchBlock) {
+      double xs[XU];
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+        const int j = j0 + u * kBatchBlock;
+        xs[u] = j < I.n ? gld_agent(&I.cs[j].x) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+        x1 += fabs(xs[u]);
+        nz += xs[u] != 0.0;
+      }
+    }
+    if (e_lo < e_hi) {
+      RefEntry nxt[U];
+      auto load = [&](int32_t base) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) nxt[u] = gld(I.rent + min(base + u * 64 + ln, e_hi - 1));
+      };
+      load(e_lo);
+      for (int32_t base = e_lo; base < e_hi; base += 64 * U) {
+        RefEntry cur[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+        load(base + 64 * U);
+        double xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = gld_agent(&I.cs[cur[u].col].x);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (base + u * 64 + ln < e_hi && xv[u] != 0.0) radd(cur[u].row, cur[u].val * xv[u]);
+      }
+    }
+  }
+  __syncthreads();
+  // g_j = <a_j, r64> for every column, streaming the entries: wave w takes
+  // the contiguous entries [E w/16, E (w+1)/16), 64 consecutive entries per
+  // step and U steps per round (the next round's loads in flight while this
+  // round is summed), sums the runs of equal column by a segmented scan
+  // across the wave and stores each column it completes.  Two runs per wave
+  // are joined by thread 0 afterwards: the wave's first column (it may have
+  // begun in the previous range) and the run still open at its end.  Columns
+  // without entries keep gref = 0.
+  {
+    double *slot = sh + 4 * wv;  // head column, head sum, tail column, tail sum
+    if (ln == 0) slot[0] = -1.0;
+    int first_col = -1, open_col = -1;
+    double open_sum = 0.0;
+    auto emit = [&](int c, double v) {
+      if (c == first_col) {
+        slot[0] = (double)c;
+        slot[1] = v;
+      } else {
+        *gp(&I.cs[c].gref) = v;
+        ginf = fmax(ginf, fabs(v));
+      }
+    };
+    RefEntry nxt[U];
+    // every lane loads; a lane past the range re-reads the range's last entry
+    // and is masked where the entry is used
+    auto load = [&](int32_t base) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) nxt[u] = gld(I.rent + min(base + u * 64 + ln, e_hi - 1));
+    };
+    if (e_lo < e_hi) {
+      load(e_lo);
+      first_col = __builtin_amdgcn_readfirstlane(nxt[0].col);
+      for (int32_t base = e_lo; base < e_hi; base += 64 * U) {
+        RefEntry cur[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+        load(base + 64 * U);
+        double rv[U];  // the round's gathers, all issued before the first is used
+#pragma unroll
+        for (int u = 0; u < U; ++u) rv[u] = rget(cur[u].row);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t cb = base + u * 64;
+          if (cb >= e_hi) break;  // wave-uniform
+          const bool in = cb + ln < e_hi;
+          const int key = in ? cur[u].col : 0x7fffffff;
+          const double p = in ? cur[u].val * rv[u] : 0.0;
+          const SegSum r = seg_scan_sum(p, key);
+          double v = r.sum;
+          const int k0 = __builtin_amdgcn_readfirstlane(key);
+          if (open_col >= 0) {
+            if (k0 == open_col) {
+              if (key == open_col) v += open_sum;
+            } else if (ln == 0) {
+              emit(open_col, open_sum);  // the open run ended with the last step
+            }
+          }
+          const int lv = min(63, e_hi - 1 - cb);  // last lane in range
+          const uint64_t heads = __ballot(r.head);
+          if (ln < lv && ((heads >> (ln + 1)) & 1)) emit(key, v);
+          open_col = __builtin_amdgcn_readlane(key, lv);
+          const long long vb = __double_as_longlong(v);
+          const int lo = __builtin_amdgcn_readlane((int)vb, lv);
+          const int hi = __builtin_amdgcn_readlane((int)(vb >> 32), lv);
+          open_sum = __longlong_as_double(
+              (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+        }
+      }
+    }
+    if (ln == 0) {
+      slot[2] = (double)open_col;
+      slot[3] = open_sum;
+    }
+    __syncthreads();
+    if (tid == 0) {  // join the partial runs, in entry order
+      int c = -1;
+      double v = 0.0;
+      for (int k = 0; k < 2 * kBatchWaves; ++k) {
+        const int kc = (int)sh[2 * k];
+        if (kc < 0) continue;
+        if (kc == c) {
+          v += sh[2 * k + 1];
+          continue;
+        }
+        if (c >= 0) {
+          *gp(&I.cs[c].gref) = v;
+          ginf = fmax(ginf, fabs(v));
+        }
+        c = kc;
+        v = sh[2 * k + 1];
+      }
+      if (c >= 0) {
+        *gp(&I.cs[c].gref) = v;
+        ginf = fmax(ginf, fabs(v));
+      }
+    }
+    __syncthreads();  // sh is free for block_stats
   }
   double r2 = 0, br = 0;
   for (int k = tid; k < I.m; k += kBatchBlock) {
@@ -293,6 +415,18 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
     out->rel_gap = P != 0 ? (P - D) / fabs(P) : 0.0;
   }
   __syncthreads();  // gref, dr and *out are settled for the steps
+}
+
+// r64 in LDS when its m doubles fit (then dr fits as well); one body per
+// case, so that no access is issued for the case not taken
+template <typename T, bool LDSR>
+__device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double lambda, T *lds,
+                                           double *sh, int lds_doubles, BatchResult *out,
+                                           bool dense) {
+  if (LDSR && Ip->m <= lds_doubles)
+    batch_refresh_impl<T, LDSR, LDSR>(Ip, lambda, lds, sh, out, dense);
+  else
+    batch_refresh_impl<T, LDSR, false>(Ip, lambda, lds, sh, out, dense);
 }
 
 // nsteps synchronous steps of one instance; S slots per G-lane group.  The
@@ -393,11 +527,16 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
     const BatchInstance *Ip = a.inst + i;
     const BatchInstance I = *Ip;
     const bool in_lds = I.drg == nullptr;
+    // the scatter streams every entry once the last refresh of this
+    // instance found more than n/64 nonzero x_j (where streaming all entries
+    // costs less than the columns' dependent loads, measured)
+    bool dense = false;
     auto refresh = [&](double lambda) {
       if (in_lds)
-        batch_refresh<T, true>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+        batch_refresh<T, true>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
       else
-        batch_refresh<T, false>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+        batch_refresh<T, false>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
+      dense = 64.0 * s_res.nnz_x > (double)I.n;
     };
     const uint64_t steps_per_epoch = ((uint64_t)I.n + I.tau - 1) / I.tau;
     uint64_t step = 0;
@@ -567,11 +706,16 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
     const BatchInstance *Ip = a.inst + i;
     const BatchInstance I = *Ip;
     const bool in_lds = I.drg == nullptr;
+    // the scatter streams every entry once the last refresh of this
+    // instance found more than n/64 nonzero x_j (where streaming all entries
+    // costs less than the columns' dependent loads, measured)
+    bool dense = false;
     auto refresh = [&](double lambda) {
       if (in_lds)
-        batch_refresh<T, true>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+        batch_refresh<T, true>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
       else
-        batch_refresh<T, false>(Ip, lambda, lds, sh, a.lds_doubles, &s_res);
+        batch_refresh<T, false>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
+      dense = 64.0 * s_res.nnz_x > (double)I.n;
     };
     uint64_t k = 0;  // updates per worker so far
     for (int l = 0; l < a.path_len; ++l) {

@@ -193,6 +193,71 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// DPP move in which lanes of rows outside ROWS, and lanes whose source lies
+// outside their row (row shifts) or the wave (wave shifts), keep `old`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_keep(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_keep(double old, double v) {
+  const long long o = __double_as_longlong(old), b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Segmented inclusive sum across one 64-wide wavefront: lane l receives the
+// sum of v over the lanes l' <= l whose key equals its own, for keys that
+// never decrease with the lane index (equal keys are then contiguous).
+// Hillis-Steele steps of 1, 2, 4, 8 lanes inside each row of 16 (DPP row
+// shifts), then the last lane of row 0 and of row 2 into the row after it
+// (row_bcast:15), then lane 31 into rows 2 and 3 (row_bcast:31): after the
+// steps inside rows, lane l holds the run from max(start of its key, start of
+// its row), and each broadcast extends runs whose key continues across the
+// row boundary.  `head` is true where the key differs from the previous
+// lane's (lane 0 always), read off the same moves.
+struct SegSum {
+  double sum;
+  bool head;
+};
+__device__ __forceinline__ SegSum seg_scan_sum(double v, int key) {
+  constexpr int kNone = -2;  // keys are >= -1
+  constexpr int kShr1 = 0x111, kShr2 = 0x112, kShr4 = 0x114, kShr8 = 0x118;
+  constexpr int kBcast15 = 0x142, kBcast31 = 0x143;
+  const int lane = threadIdx.x & 63;
+  int prev;
+  {
+    const int k2 = dpp_keep<kShr1, 0xF>(kNone, key);
+    const double v2 = dpp_keep<kShr1, 0xF>(0.0, v);
+    prev = k2;
+    if (k2 == key) v += v2;
+  }
+#define ACDC_SEG_STEP(CTRL, ROWS)                        \
+  {                                                      \
+    const int k2 = dpp_keep<CTRL, ROWS>(kNone, key);     \
+    const double v2 = dpp_keep<CTRL, ROWS>(0.0, v);      \
+    if (k2 == key) v += v2;                              \
+  }
+  ACDC_SEG_STEP(kShr2, 0xF)
+  ACDC_SEG_STEP(kShr4, 0xF)
+  ACDC_SEG_STEP(kShr8, 0xF)
+#undef ACDC_SEG_STEP
+  {
+    const int k2 = dpp_keep<kBcast15, 0xA>(kNone, key);  // lane 15 -> row 1, lane 47 -> row 3
+    const double v2 = dpp_keep<kBcast15, 0xA>(0.0, v);
+    if (lane == 16 || lane == 48) prev = k2;
+    if (k2 == key) v += v2;
+  }
+  {
+    const int k2 = dpp_keep<kBcast31, 0xC>(kNone, key);  // lane 31 -> rows 2 and 3
+    const double v2 = dpp_keep<kBcast31, 0xC>(0.0, v);
+    if (lane == 32) prev = k2;
+    if (k2 == key) v += v2;
+  }
+  return SegSum{v, lane == 0 || prev != key};
+}
+
 template <int G, typename T>
 __device__ __forceinline__ T group_max(T v) {
 #pragma unroll

@@ -109,21 +109,44 @@ __device__ __forceinline__ double coord_delta(const StepArgs<T> &a, uint32_t j,
 // Synchronous step s, gather half: group -> slot i < tau, coordinate
 // j = perm_s(i); g = <segment j, residual>; delta from the loss's proximal
 // rule; x_j += delta (coordinates of one step are distinct, so no other group
-// touches x_j); the scatter coefficient is staged for phase B.
+// touches x_j); the scatter coefficient is staged for phase B.  A group is 8
+// to 64 lanes of one wave, or the whole 256-thread block for power-law-long
+// segments (SVM sample rows), where the longest segment of a step would
+// otherwise keep one wave walking it while the rest of the step waits.
 // Restates compute_update + the x half of performOneCoordinateUpdate
 // (MulticoreL1RegSquareLoss.h:41-56, MulticoreSVMDualLoss.h:119-134) with the
 // synchronous semantics of cdn_common.h:255-270.
 template <typename T, int G, int LOSS>
 __global__ __launch_bounds__(kBlock) void k_step_gather(StepArgs<T> a, uint64_t step) {
+  static_assert(G <= 64 || G == kBlock, "a group is at most a wave, or the whole block");
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t slot = t / G;
   const int lane = threadIdx.x & (G - 1);
   if (slot >= a.tau) return;  // whole groups leave together
   const uint32_t j = a.perm((uint32_t)slot, step_keys(a.seed, step));
   const int32_t p0 = a.A.ptr[j], p1 = a.A.ptr[j + 1];
+  // four entries per lane in flight: their indices, then their gathers
   T acc = 0;
-  for (int32_t p = p0 + lane; p < p1; p += G) acc += a.A.val[p] * a.res[a.A.idx[p]];
-  acc = group_sum<G>(acc);
+  int32_t p = p0 + lane;
+  for (; p + 3 * G < p1; p += 4 * G) {
+    int32_t r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = a.A.idx[p + u * G];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += a.A.val[p + u * G] * a.res[r[u]];
+  }
+  for (; p < p1; p += G) acc += a.A.val[p] * a.res[a.A.idx[p]];
+  if constexpr (G == kBlock) {  // a block per coordinate: the waves meet in LDS
+    __shared__ T part[kBlock / 64];
+    acc = group_sum<64>(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    acc = part[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) acc += part[w];
+  } else {
+    acc = group_sum<G>(acc);
+  }
   const double xj = a.x[j];
   double c;
   const double d = coord_delta<T, LOSS>(a, j, (double)acc, xj, &c);
@@ -150,7 +173,19 @@ __global__ __launch_bounds__(kBlock) void k_step_scatter(StepArgs<T> a) {
   const int32_t j = a.coord[slot];
   const T ct = (T)c;
   const int32_t p0 = a.A.ptr[j], p1 = a.A.ptr[j + 1];
-  for (int32_t p = p0 + lane; p < p1; p += G) unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
+  int32_t p = p0 + lane;
+  for (; p + 3 * G < p1; p += 4 * G) {
+    int32_t r[4];
+    T v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      r[u] = a.A.idx[p + u * G];
+      v[u] = a.A.val[p + u * G];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) unsafeAtomicAdd(&a.res[r[u]], ct * v[u]);
+  }
+  for (; p < p1; p += G) unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
 }
 
 // ------------------------------------------------------ grid-wide barrier

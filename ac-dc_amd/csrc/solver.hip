@@ -271,8 +271,17 @@ int auto_lanes(int64_t nnz, int64_t nseg) {
   return 64;
 }
 
-// ---- dispatch: value type x lanes x loss
-template <template <typename, int, int> class F, typename... Args>
+// synchronous steps: a whole block per coordinate once segments average more
+// than 192 entries (the SVM's lognormal sample rows: a step waits for its
+// longest row, and 256 lanes walk it four times faster than one wave)
+int auto_step_lanes(int64_t nnz, int64_t nseg) {
+  const double avg = nseg > 0 ? (double)nnz / (double)nseg : 0.0;
+  return avg > 192 ? kBlock : auto_lanes(nnz, nseg);
+}
+
+// ---- dispatch: value type x lanes x loss (WIDE: also a block per coordinate,
+// for the per-step kernels only)
+template <template <typename, int, int> class F, bool WIDE = false, typename... Args>
 void dispatch3(const acdc_solver *s, int lanes, Args &&...args) {
   auto go = [&](auto tag_t, auto tag_g) {
     using T = typename decltype(tag_t)::type;
@@ -288,6 +297,12 @@ void dispatch3(const acdc_solver *s, int lanes, Args &&...args) {
       case 8: go(tag_t, std::integral_constant<int, 8>{}); break;
       case 16: go(tag_t, std::integral_constant<int, 16>{}); break;
       case 32: go(tag_t, std::integral_constant<int, 32>{}); break;
+      case kBlock:
+        if constexpr (WIDE) {
+          go(tag_t, std::integral_constant<int, kBlock>{});
+          break;
+        }
+        [[fallthrough]];
       default: go(tag_t, std::integral_constant<int, 64>{}); break;
     }
   };
@@ -486,7 +501,7 @@ acdc_status run_persistent(acdc_solver *s, uint64_t step0, uint64_t nsteps) {
         return fail(ACDC_ERR_HIP, std::string("cooperative launch: ") + hipGetErrorString(err));
       (void)hipGetLastError();
       s->persistent = false;
-      for (uint64_t k = done; k < nsteps; ++k) dispatch3<LaunchStep>(s, s->lanes, step0 + k, 3);
+      for (uint64_t k = done; k < nsteps; ++k) dispatch3<LaunchStep, true>(s, s->lanes, step0 + k, 3);
       return ACDC_OK;
     }
     done += chunk;
@@ -514,8 +529,14 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
   if (set->launch_mode < ACDC_LAUNCH_AUTO || set->launch_mode > ACDC_LAUNCH_PERSISTENT)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown launch mode");
   if (set->lanes_per_coord != 0 && set->lanes_per_coord != 8 && set->lanes_per_coord != 16 &&
-      set->lanes_per_coord != 32 && set->lanes_per_coord != 64)
-    return fail(ACDC_ERR_INVALID, "acdc_solver_create: lanes_per_coord must be 0, 8, 16, 32 or 64");
+      set->lanes_per_coord != 32 && set->lanes_per_coord != 64 && set->lanes_per_coord != kBlock)
+    return fail(ACDC_ERR_INVALID,
+                "acdc_solver_create: lanes_per_coord must be 0, 8, 16, 32, 64 or 256");
+  if (set->lanes_per_coord == kBlock &&
+      (set->step_mode == ACDC_STEP_ASYNC || set->launch_mode == ACDC_LAUNCH_PERSISTENT))
+    return fail(ACDC_ERR_INVALID,
+                "acdc_solver_create: 256 lanes per coordinate (a block per coordinate) is for "
+                "synchronous steps with per-step launches");
   if (!(set->lambda >= 0) || (loss == kSvm && !(set->lambda > 0)))
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: lambda must be >= 0 (> 0 for the SVM)");
   if (set->refresh_period < 0)
@@ -548,7 +569,11 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
   s->nnz = p->nnz;
   s->beta = beta;
   s->fp32 = set->precision == ACDC_PREC_FP32_REFRESH;
-  s->lanes = set->lanes_per_coord ? set->lanes_per_coord : auto_lanes(p->nnz, p->ncoord);
+  s->lanes = set->lanes_per_coord ? set->lanes_per_coord
+                                   : (set->step_mode == ACDC_STEP_SYNC &&
+                                              set->launch_mode != ACDC_LAUNCH_PERSISTENT
+                                          ? auto_step_lanes(p->nnz, p->ncoord)
+                                          : auto_lanes(p->nnz, p->ncoord));
   s->eval_lanes = auto_lanes(p->nnz, p->ncoord);
 
   auto bail = [&](acdc_status e) {
@@ -614,7 +639,7 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
   s->async = set->step_mode == ACDC_STEP_ASYNC;
   for (size_t c = 0; c + 1 < p->cptr.size(); ++c)
     s->maxlen = std::max<int64_t>(s->maxlen, (int64_t)p->cptr[c + 1] - p->cptr[c]);
-  if (!s->async && set->tau >= 2 && set->launch_mode != ACDC_LAUNCH_PER_STEP) {
+  if (!s->async && set->tau >= 2 && set->launch_mode != ACDC_LAUNCH_PER_STEP && s->lanes != kBlock) {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     (void)hipGetDevice(&dev);
@@ -703,7 +728,7 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
       if ((st = run_persistent(s, s->step, chunk)) != ACDC_OK) return st;
       s->step_launches += s->persistent ? (chunk + kStepsPerLaunch - 1) / kStepsPerLaunch : 2 * chunk;
     } else {
-      for (uint64_t k = 0; k < chunk; ++k) dispatch3<LaunchStep>(s, s->lanes, s->step + k, 3);
+      for (uint64_t k = 0; k < chunk; ++k) dispatch3<LaunchStep, true>(s, s->lanes, s->step + k, 3);
       s->step_launches += 2 * chunk;
     }
     if ((st = step_mark(s)) != ACDC_OK) return st;
@@ -925,9 +950,9 @@ extern "C" acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps
   if (st == ACDC_OK) {
     for (uint64_t k = 0; k < nsteps; ++k) {
       (void)hipEventRecord(ev[3 * k], s->stream);
-      dispatch3<LaunchStep>(s, s->lanes, s->step, 1);
+      dispatch3<LaunchStep, true>(s, s->lanes, s->step, 1);
       (void)hipEventRecord(ev[3 * k + 1], s->stream);
-      dispatch3<LaunchStep>(s, s->lanes, s->step, 2);
+      dispatch3<LaunchStep, true>(s, s->lanes, s->step, 2);
       (void)hipEventRecord(ev[3 * k + 2], s->stream);
       s->step += 1;
       s->updates += (uint64_t)s->set.tau;

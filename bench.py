@@ -34,16 +34,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def profiled_traffic():
-    """HBM bytes per k_async launch on this workload from the committed
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by a copy of known size),
-    or None when the summary is absent."""
+    or None when the summary has no line for that kernel."""
     import re
     path = os.path.join(ROOT, "profiles", "lasso_c2_async_traffic.md")
     try:
-        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.]+)", open(path).read(), re.M)
+        text = open(path).read()
     except OSError:
         return None
+    m = re.search(r"^traffic_bytes_per_launch\[" + re.escape(kernel) + r"\]:\s*([0-9.]+)", text, re.M)
     return float(m.group(1)) if m else None
 
 
@@ -226,7 +227,7 @@ def ttd_c4(acdc, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100, help="timed epochs")
+    ap.add_argument("--steps", type=int, default=50, help="timed epochs")
     ap.add_argument("--warmup", type=int, default=2, help="untimed epochs")
     ap.add_argument("--tau", type=int, default=4096)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per coordinate (0 = autotune)")
@@ -291,6 +292,8 @@ def main():
     bytes_per_update = 8.0 * mean_nnz + 32.0  # (row, value) per nnz; ptr pair, L_j, gref_j, x_j
     bytes_per_launch = upd_per_launch * bytes_per_update
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    # segments longer than the R = 4 register chunks of a group take the TAIL variant
+    kernel = f"k_async<float,{args.lanes},0,4,{'true' if maxlen > 4 * args.lanes else 'false'}>"
     log(f"[bench] k_async: {launch_ms:.3f} ms per launch ({upd_per_launch:.0f} updates), "
         f"{achieved:.1f} GB/s")
 
@@ -306,8 +309,8 @@ def main():
                                 "fp64 refresh every 5 epochs",
                    "step": "one epoch = n coordinate updates, from x = 0"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
-                     "kernel": f"k_async<float,{args.lanes},0,4,{'true' if maxlen > 4 * args.lanes else 'false'}>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kernel),
+                     "kernel": kernel,
                      "launch_ms": launch_ms,
                      "updates_per_launch": upd_per_launch,
                      "bytes_per_update": bytes_per_update, "bytes_per_launch": bytes_per_launch},

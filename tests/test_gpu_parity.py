@@ -85,10 +85,11 @@ def test_tau1_trajectory_fp64(acdc, orc, c1):
     s.close()
 
 
-@pytest.mark.parametrize("lanes,launch", [(8, 1), (16, 1), (32, 1), (64, 1),
+@pytest.mark.parametrize("lanes,launch", [(8, 1), (16, 1), (32, 1), (64, 1), (256, 1),
                                           (8, 2), (16, 2), (32, 2), (64, 2)])
 def test_sync_tau_fp64_matches_oracle(acdc, orc, c1, lanes, launch):
-    """launch 1: one gather + one scatter launch per step; 2: persistent grid."""
+    """launch 1: one gather + one scatter launch per step; 2: persistent grid;
+    256 lanes: a whole block per coordinate."""
     tau = 512
     p = c1.problem()
     s = acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=c1.lam, tau=tau, seed=3, lanes_per_coord=lanes,
@@ -189,8 +190,11 @@ def test_svm_matches_oracle(acdc, orc):
     d = acdc.gen_svm(2000, 50_000, row_median=300.0, row_sigma=0.901, wstar_nnz=1000,
                      flip_frac=0.1, seed=3)
     p = d.problem()
-    for tau in (1, 256):
-        s = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=tau, seed=9)
+    # automatic lanes: a block per sample row (rows average ~400 entries); 64:
+    # one wave per row
+    for tau, lanes in ((1, 0), (256, 0), (256, 64)):
+        s = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=tau, seed=9,
+                        lanes_per_coord=lanes)
         o = orc.SVM(d.n, d.m, d.ptr, d.idx, d.vals, d.rhs, d.lam, beta=s.beta)
         steps = 2 * d.n // tau
         s.run_updates(steps * tau)
