@@ -169,9 +169,16 @@ struct DrView {
   T *lds;
   T *glob;
   int32_t split;
+  // both paths into their own registers, then a select: a single
+  // destination written by an LDS read in some lanes and a global load in
+  // others would make every LDS read wait for all global loads in flight
   __device__ __forceinline__ T get(int32_t row) const {
-    if (FULL || row < split) return lds[row];
-    return gld_agent(&glob[row - split]);
+    if (FULL) return lds[row];
+    const bool here = row < split;
+    const T a = lds[here ? row : 0];
+    T b = (T)0;
+    if (!here) b = gld_agent(&glob[row - split]);
+    return here ? a : b;
   }
   __device__ __forceinline__ void add(int32_t row, T v) const {
     if (FULL || row < split)
@@ -251,10 +258,15 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
         nz += 1.0;
         const int j = j0 + u * kBatchBlock;
         const int32_t p0 = gld(&I.cs[j].p0), len = gld(&I.cs[j].len);
-#pragma unroll 4
-        for (int32_t q = 0; q < len; ++q) {
-          const RefEntry e = gld(I.rent + p0 + q);
-          radd(e.row, e.val * xs[u]);
+        // four entries loaded before their adds (with r64 in global memory
+        // the adds are atomics, and a load issued after one waits for it)
+        for (int32_t q0 = 0; q0 < len; q0 += 4) {
+          RefEntry e[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) e[q] = gld(I.rent + p0 + min(q0 + q, len - 1));
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (q0 + q < len) radd(e[q].row, e[q].val * xs[u]);
         }
       }
     }
@@ -630,6 +642,12 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
   };
   auto gather = [&](int32_t row) -> T { return dr.get(row); };
   auto update = [&](uint64_t k, Slot &cur, Slot &nxt, Slot &ahead) {
+    // gathers of every lane, without branches (padding lanes read row 0 and
+    // weigh it by 0), issued first: residual rows in global memory then
+    // return ahead of the next updates' loads, which stay in flight
+    T v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = gather(cur.ci[r]);
     entries(nxt);        // update k+1 (past the last update: harmless loads)
     record(k + 2, ahead);
     // x_j from the record loaded two updates ago, plus this worker's own
@@ -640,11 +658,6 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
     double xj = cur.r.x;
     if (cur.j == j2) xj += d2;
     if (cur.j == j1) xj += d1;
-    // gathers of every lane, without branches (padding lanes read row 0 and
-    // weigh it by 0), all issued before any is used
-    T v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = gather(cur.ci[r]);
     T acc = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) acc += cur.cv[r] * v[r];
