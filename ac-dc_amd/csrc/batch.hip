@@ -36,6 +36,14 @@ acdc_status bfail(acdc_status st, const std::string &msg) {
       return bfail(ACDC_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));     \
   } while (0)
 
+// inside acdc_batch_create: a failing call also frees the partial batch
+#define CREATE_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return bail(bfail(ACDC_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_))); \
+  } while (0)
+
 constexpr size_t kLdsBytes = 160 * 1024 - 1024;  // dynamic LDS per block (static scratch aside)
 // coordinates per lane group and step; with 4 register chunks per column
 // this keeps every instantiation within 128 VGPRs (1024-thread blocks)
@@ -248,6 +256,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   // entry unconditionally (from the segment start, which for an empty last
   // segment is the pad)
   if ((st = balloc(b, &dent, (tot_nnz + 1) * esz)) != ACDC_OK) return bail(st);
+  if (hipMemset(dent + tot_nnz * esz, 0, esz) != hipSuccess)  // the pad entry: row 0, value 0
+    return bail(bfail(ACDC_ERR_HIP, "acdc_batch_create: pad entry"));
   if ((st = balloc(b, &drent, tot_nnz)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &db, tot_m)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &dr64, tot_m)) != ACDC_OK) return bail(st);
@@ -346,31 +356,31 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     return (double)problems[i]->nnz * (problems[i]->nres <= lds_rows ? 1.0 : 1.3);
   };
   std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) { return cost(a) > cost(c); });
-  BHIP(hipMemcpy(b->dorder, ord.data(), count * 4, hipMemcpyHostToDevice));
-  BHIP(hipMemcpy(b->dinst, hinst.data(), count * sizeof(BatchInstance), hipMemcpyHostToDevice));
-  BHIP(hipMemcpy(b->dlambda, lambdas, (size_t)count * path_len * 8, hipMemcpyHostToDevice));
+  CREATE_HIP(hipMemcpy(b->dorder, ord.data(), count * 4, hipMemcpyHostToDevice));
+  CREATE_HIP(hipMemcpy(b->dinst, hinst.data(), count * sizeof(BatchInstance), hipMemcpyHostToDevice));
+  CREATE_HIP(hipMemcpy(b->dlambda, lambdas, (size_t)count * path_len * 8, hipMemcpyHostToDevice));
 
   // grid sizes: resident blocks of each variant
   int dev = 0;
   hipDeviceProp_t prop;
-  BHIP(hipGetDevice(&dev));
-  BHIP(hipGetDeviceProperties(&prop, dev));
+  CREATE_HIP(hipGetDevice(&dev));
+  CREATE_HIP(hipGetDeviceProperties(&prop, dev));
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
     b->clock_khz = khz;
   {
     const size_t shmem = (kLdsBytes / tsz) * tsz;
-    BHIP(hipFuncSetAttribute(batch_kernel(b), hipFuncAttributeMaxDynamicSharedMemorySize,
+    CREATE_HIP(hipFuncSetAttribute(batch_kernel(b), hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)shmem));
     int per_cu = 0;
-    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(b), kBatchBlock,
+    CREATE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(b), kBatchBlock,
                                                       shmem));
     b->blocks = std::max(1, per_cu) * prop.multiProcessorCount;
   }
-  BHIP(hipStreamSynchronize(b->stream));
+  CREATE_HIP(hipStreamSynchronize(b->stream));
 
   if (set->use_graph) {
-    BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    CREATE_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     st = enqueue_run(b);
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(b->stream, &g);
@@ -378,7 +388,7 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     if (ce != hipSuccess)
       return bail(bfail(ACDC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce)));
     b->graph = g;
-    BHIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    CREATE_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
   *out = b;
   return ACDC_OK;

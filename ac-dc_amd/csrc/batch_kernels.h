@@ -630,21 +630,25 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
   // the pad entry after the last segment), so that the number of memory
   // instructions in flight is the same on every path and the wait before
   // an update's gathers only covers loads issued one update earlier
+  // (a padding lane keeps the entry it loaded and is masked where the entry
+  // is used -- its row is clamped into [0, m) for the gather, since an empty
+  // segment's start is the next instance's first entry or the pad entry --
+  // a select on the loaded value here would wait for the load at once)
   auto entries = [&](Slot &c) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int32_t q = lane + r * G;
-      const bool in = q < c.r.len;
-      const Entry<T> e = gld(ent + c.r.p0 + (in ? q : 0));
-      c.ci[r] = in ? e.row : 0;
-      c.cv[r] = in ? e.val : (T)0;
+      const Entry<T> e = gld(ent + c.r.p0 + (q < c.r.len ? q : 0));
+      c.ci[r] = e.row;
+      c.cv[r] = e.val;
     }
   };
-  auto gather = [&](int32_t row) -> T { return dr.get(row); };
+  auto gather = [&](int32_t row) -> T { return dr.get((uint32_t)row < (uint32_t)I.m ? row : 0); };
   auto update = [&](uint64_t k, Slot &cur, Slot &nxt, Slot &ahead) {
-    // gathers of every lane, without branches (padding lanes read row 0 and
-    // weigh it by 0), issued first: residual rows in global memory then
-    // return ahead of the next updates' loads, which stay in flight
+    // gathers of every lane, without branches (padding lanes read a valid
+    // row and are masked below), issued first: residual rows in global
+    // memory then return ahead of the next updates' loads, which stay in
+    // flight
     T v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = gather(cur.ci[r]);
@@ -660,7 +664,7 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
     if (cur.j == j1) xj += d1;
     T acc = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc += cur.cv[r] * v[r];
+    for (int r = 0; r < R; ++r) acc += lane + r * G < cur.r.len ? cur.cv[r] * v[r] : (T)0;
     if (TAIL)
       for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
         const Entry<T> e = gld(ent + cur.r.p0 + q);
