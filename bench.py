@@ -48,6 +48,69 @@ def profiled_traffic(kernel):
     return float(m.group(1)) if m else None
 
 
+def profiled_per_launch(kernel, key):
+    """A per-launch figure of `kernel` from the committed rocprofv3 counter
+    summary of this workload (lines `<key>[<kernel>]: <value>`), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "lasso_c2_async_atomics.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^" + key + r"\[" + re.escape(kernel) + r"\]:\s*([0-9.]+)", text, re.M)
+    return float(m.group(1)) if m else None
+
+
+def device_probes():
+    """Two rates of this device, measured in a child process with plain
+    PyTorch operations (PyTorch brings its own HIP runtime, which cannot
+    start in a process where the library's runtime already runs): see
+    probe_device.  Returns (None, None) when the child fails."""
+    import subprocess
+    try:
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--probe-device"],
+                             capture_output=True, text=True, timeout=300)
+        line = [l for l in out.stdout.splitlines() if l.startswith("{")][-1]
+        j = json.loads(line)
+        return j["copy_gbs"], j["atomics_per_s"]
+    except (subprocess.SubprocessError, IndexError, ValueError, KeyError) as e:
+        log(f"[bench] device probes failed: {e}")
+        return None, None
+
+
+def probe_device():
+    """The achievable HBM streaming rate (five copies of a 2 GiB buffer, read
+    + write bytes over their HIP-event time) and the random FP32 atomic-add
+    rate (index_add_ of 2^26 ones at uniform random rows of a 200,000-row
+    array, the size of the 200,000 x 1,000,000 Lasso's residual)."""
+    import torch
+    dev = torch.device("cuda:0")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a = torch.ones(1 << 28, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * a.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    r = torch.zeros(200_000, dtype=torch.float32, device=dev)
+    idx = torch.randint(0, r.numel(), (1 << 26,), device=dev)
+    v = torch.ones(1 << 26, dtype=torch.float32, device=dev)
+    r.index_add_(0, idx, v)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        r.index_add_(0, idx, v)
+    e1.record()
+    torch.cuda.synchronize()
+    atom = 5 * idx.numel() / (e0.elapsed_time(e1) * 1e-3)
+    return copy_gbs, atom
+
+
 def time_to_gap(solver, n, target, check_every, max_epochs):
     """Wall time from x = 0 to rel gap <= target: one run_updates per epoch,
     the FP64 gap every check_every epochs (evaluations included)."""
@@ -241,6 +304,10 @@ def main():
     ap.add_argument("--no-ttg", action="store_true", help="skip the time-to-gap runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-svm-ttd", action="store_true", help="skip the SVM and TTD workloads")
+    ap.add_argument("--no-probes", action="store_true",
+                    help="skip the streaming-copy and atomic-rate probes (a child process "
+                         "running PyTorch kernels)")
+    ap.add_argument("--probe-device", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-path", type=int, default=32)
     ap.add_argument("--batch-max-epochs", type=int, default=100)
@@ -250,6 +317,10 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
               "run without --gpus or with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.probe_device:  # the child of device_probes: PyTorch only
+        copy_gbs, atom = probe_device()
+        print(json.dumps({"copy_gbs": copy_gbs, "atomics_per_s": atom}), flush=True)
+        return
 
     import numpy as np
     import acdc_amd as acdc
@@ -296,6 +367,11 @@ def main():
     kernel = f"k_async<float,{args.lanes},0,4,{'true' if maxlen > 4 * args.lanes else 'false'}>"
     log(f"[bench] k_async: {launch_ms:.3f} ms per launch ({upd_per_launch:.0f} updates), "
         f"{achieved:.1f} GB/s")
+    copy_gbs, atomic_probe = (None, None) if args.no_probes else device_probes()
+    if not args.no_probes:
+        log(f"[bench] device: {copy_gbs:.0f} GB/s streaming copy, {atomic_probe:.3e} random FP32 "
+            f"atomic adds/s")
+    atomics_per_launch = profiled_per_launch(kernel, "atomics_per_launch")
 
     result = {
         "metric": METRIC, "value": value, "unit": "coord updates/s", "n_gpus": 1,
@@ -310,10 +386,21 @@ def main():
                    "step": "one epoch = n coordinate updates, from x = 0"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kernel),
-                     "kernel": kernel,
+                     "kernel": kernel, "copy_gbs": copy_gbs,
                      "launch_ms": launch_ms,
                      "updates_per_launch": upd_per_launch,
                      "bytes_per_update": bytes_per_update, "bytes_per_launch": bytes_per_launch},
+        # updates/s inside the update kernels alone (HIP events), beside `value`,
+        # which also holds the refreshes every 5 epochs and the host loop
+        "kernel_only": {"value": upd_per_launch / (launch_ms * 1e-3), "unit": "coord updates/s"},
+        # the residual scatter's L2 atomics: per launch from the committed
+        # counter summary, over this run's launch time, against the random
+        # FP32 atomic rate of the device measured just before
+        "atomics": {"per_s": (atomics_per_launch / (launch_ms * 1e-3)
+                              if atomics_per_launch else None),
+                    "probe_random_fp32_per_s": atomic_probe,
+                    "frac": (atomics_per_launch / (launch_ms * 1e-3) / atomic_probe
+                             if atomics_per_launch and atomic_probe else None)},
     }
 
     # ---- plain FP64 path: synchronous steps, per-step launches
