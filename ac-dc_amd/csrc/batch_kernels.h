@@ -596,9 +596,10 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
 // them -- LDS atomics on the residual correction, atomic adds on x_j.  Only
 // the FP64 refresh, every refresh_period epochs, stops all workers (one
 // block barrier).  An update reads its 32-byte record (x_j included) two
-// updates ahead and its entries one update ahead, so its critical path is
-// the LDS gather: three or four memory requests per update, the resource a
-// CU runs out of first.
+// updates ahead and its entries one update ahead; its gathers go out before
+// those loads, so that waiting for them does not wait for the prefetches
+// (memory counters drain in issue order).  A worker is then bound by about
+// one memory round trip per update, and a CU by its workers' count.
 template <typename T, int G, int R, bool LDSR, bool TAIL>
 __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, double lambda,
                                                     uint64_t k0, uint64_t count, T *lds) {
