@@ -32,6 +32,7 @@ PREC_FP32_REFRESH = 1
 BETA_OMEGA = 0
 BETA_FORCED = 1
 BETA_NONE = 2
+BETA_SPECTRAL = 3
 STEP_SYNC = 0
 STEP_ASYNC = 1
 LAUNCH_AUTO = 0

@@ -612,6 +612,56 @@ __global__ __launch_bounds__(kBlock) void k_lipschitz(Seg<T> A, int64_t nseg, do
   if (lane == 0) L[c] = s > 0 ? 1.0 / (beta * s) : 0.0;
 }
 
+// ------------------------------------------------------ spectral ESO beta
+// Power steps on D^-1/2 A^T A D^-1/2 (segments as columns, D = diag
+// ||a_c||^2), as mSDCA.cpp:188-265: y = sum_c a_c (w_c v_c scale) with
+// w_c = 1/||a_c|| (0 for an empty segment), then v_c = w_c <a_c, y>.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_pm_scatter(Seg<double> A, int64_t nseg,
+                                                       const double *__restrict__ v,
+                                                       const double *__restrict__ w, double scale,
+                                                       double *__restrict__ y) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t c = t / G;
+  const int lane = threadIdx.x & (G - 1);
+  if (c >= nseg) return;
+  const double a = v[c] * w[c] * scale;
+  if (a == 0.0) return;
+  for (int32_t p = A.ptr[c] + lane; p < A.ptr[c + 1]; p += G)
+    unsafeAtomicAdd(&y[A.idx[p]], a * A.val[p]);
+}
+
+// v_c = w_c <a_c, y> over a grid-stride of segments; partial[block] = the
+// block's sum of v_c^2
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_pm_gather(Seg<double> A, int64_t nseg,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ w,
+                                                      double *__restrict__ v,
+                                                      double *__restrict__ partial) {
+  __shared__ double sh[kBlock / 64];
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t groups = (int64_t)gridDim.x * (kBlock / G);
+  double sq = 0;
+  for (int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G; c < nseg; c += groups) {
+    double g = 0;
+    for (int32_t p = A.ptr[c] + lane; p < A.ptr[c + 1]; p += G) g += A.val[p] * y[A.idx[p]];
+    g = group_sum<G>(g) * w[c];
+    if (lane == 0) {
+      v[c] = g;
+      sq += g * g;
+    }
+  }
+  sq = group_sum<64>(sq);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int k = 0; k < kBlock / 64; ++k) t += sh[k];
+    partial[blockIdx.x] = t;
+  }
+}
+
 // ----------------------------------------------------------------- refresh
 // r64 = -b (square losses) or 0 (SVM w), then the nonzero coordinates are
 // scattered in with FP64 atomics (column-scatter form of

@@ -412,6 +412,18 @@ struct LaunchLipschitz {
   }
 };
 
+// one power step of the spectral ESO: y = sum_c a_c w_c v_c scale (y zeroed
+// before), v_c = w_c <a_c, y>, block partials of ||v||^2
+template <typename T, int G, int LOSS>
+struct LaunchPowerStep {
+  static void run(const acdc_solver *s, double *v, const double *w, double scale, double *y,
+                  double *partial, unsigned gather_blocks) {
+    const unsigned blocks = (unsigned)div_up(s->ncoord * G, kBlock);
+    k_pm_scatter<G><<<blocks, kBlock, 0, s->stream>>>(seg64(s), s->ncoord, v, w, scale, y);
+    k_pm_gather<G><<<gather_blocks, kBlock, 0, s->stream>>>(seg64(s), s->ncoord, y, w, v, partial);
+  }
+};
+
 template <typename T, int G, int LOSS>
 struct LaunchRefreshScatter {
   static void run(const acdc_solver *s, double *target) {
@@ -490,6 +502,65 @@ void destroy_solver(acdc_solver *s) {
 // steps [step0, step0 + nsteps) in persistent launches of kStepsPerLaunch.
 // A refused cooperative launch runs nothing; in ACDC_LAUNCH_AUTO the solver
 // then switches to per-step launches (same arithmetic) for good.
+// The spectral ESO's eigenvalue (mSDCA.cpp:188-265): the largest eigenvalue
+// of D^-1/2 A^T A D^-1/2 by kPowerSteps power steps from a seeded start
+// (coordinate c: the first word of Philox4x32-10 at counter (c, c >> 32,
+// "PCPM", 0) under the solver's seed, over 2^32), normalised.  Set-up time
+// only: the temporaries are freed before returning.
+constexpr int kPowerSteps = 20;
+constexpr uint32_t kPowerTag = 0x5043504Du;  // "PCPM"
+
+acdc_status spectral_lmax(acdc_solver *s, const acdc_problem *p, double *lmax) {
+  const int64_t n = s->ncoord, m = std::max<int64_t>(s->nres, 1);
+  std::vector<double> w(std::max<int64_t>(n, 1)), v(std::max<int64_t>(n, 1));
+  double nrm = 0;
+  for (int64_t c = 0; c < n; ++c) {
+    double s2 = 0;
+    for (int32_t q = p->cptr[c]; q < p->cptr[c + 1]; ++q) s2 += p->cval[q] * p->cval[q];
+    w[c] = s2 > 0 ? 1.0 / std::sqrt(s2) : 0.0;
+    uint32_t ctr[4] = {(uint32_t)c, (uint32_t)((uint64_t)c >> 32), kPowerTag, 0u};
+    acdc::Philox::gen(ctr, (uint32_t)s->set.seed, (uint32_t)(s->set.seed >> 32));
+    v[c] = (double)ctr[0] * (1.0 / 4294967296.0);
+    nrm += v[c] * v[c];
+  }
+  nrm = std::sqrt(nrm);
+  for (int64_t c = 0; c < n; ++c) v[c] /= nrm;
+  const unsigned gblocks =
+      (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, div_up(n * s->eval_lanes, kBlock)));
+  double *dv = nullptr, *dw = nullptr, *dy = nullptr, *dpart = nullptr;
+  auto cleanup = [&]() {
+    (void)hipFree(dv);
+    (void)hipFree(dw);
+    (void)hipFree(dy);
+    (void)hipFree(dpart);
+  };
+  if (hipMalloc(&dv, v.size() * 8) != hipSuccess || hipMalloc(&dw, w.size() * 8) != hipSuccess ||
+      hipMalloc(&dy, (size_t)m * 8) != hipSuccess || hipMalloc(&dpart, (size_t)gblocks * 8) != hipSuccess) {
+    cleanup();
+    return fail(ACDC_ERR_NOMEM, "acdc_solver_create: spectral beta buffers");
+  }
+  hipError_t e = hipMemcpy(dv, v.data(), v.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dw, w.data(), w.size() * 8, hipMemcpyHostToDevice);
+  std::vector<double> part(gblocks);
+  double est = 1.0, scale = 1.0;
+  for (int it = 0; it < kPowerSteps && e == hipSuccess; ++it) {
+    e = hipMemsetAsync(dy, 0, (size_t)m * 8, s->stream);
+    if (e != hipSuccess) break;
+    dispatch3<LaunchPowerStep>(s, s->eval_lanes, dv, dw, scale, dy, dpart, gblocks);
+    e = hipMemcpyAsync(part.data(), dpart, (size_t)gblocks * 8, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    double sq = 0;
+    for (double q : part) sq += q;
+    est = std::sqrt(sq);
+    scale = est > 0 ? 1.0 / est : 0.0;
+  }
+  cleanup();
+  if (e != hipSuccess)
+    return fail(ACDC_ERR_HIP, std::string("acdc_solver_create: spectral beta: ") + hipGetErrorString(e));
+  *lmax = est;
+  return ACDC_OK;
+}
+
 acdc_status run_persistent(acdc_solver *s, uint64_t step0, uint64_t nsteps) {
   for (uint64_t done = 0; done < nsteps;) {
     const uint64_t chunk = std::min(kStepsPerLaunch, nsteps - done);
@@ -554,6 +625,7 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
       beta = set->forced_beta;
       break;
     case ACDC_BETA_NONE: beta = 1.0; break;
+    case ACDC_BETA_SPECTRAL: beta = 1.0; break;  // after the upload: spectral_lmax
     default: return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown beta_mode");
   }
   int32_t ndev = 0;
@@ -660,6 +732,13 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
       return bail(fail(ACDC_ERR_INVALID,
                        "acdc_solver_create: tau * lanes_per_coord exceeds the co-resident grid "
                        "of the persistent kernel"));
+  }
+  if (set->beta_mode == ACDC_BETA_SPECTRAL) {
+    double lmax = 1.0;
+    if ((st = spectral_lmax(s, p, &lmax)) != ACDC_OK) return bail(st);
+    s->beta = s->ncoord > 1 ? 1.0 + (double)(set->tau - 1) * (lmax - 1.0) /
+                                        ((double)s->ncoord - 1.0)
+                            : 1.0;
   }
   dispatch3<LaunchLipschitz>(s, s->eval_lanes);
   if (s->async) {

@@ -96,7 +96,7 @@ class HipEngineExecutor : public AbstractEngineExecutor<L, D> {
       s.beta_mode = ACDC_BETA_FORCED;
       s.forced_beta = (double)instance.sigma;
     } else {
-      s.beta_mode = ACDC_BETA_OMEGA;
+      s.beta_mode = _settings->hipSpectralBeta ? ACDC_BETA_SPECTRAL : ACDC_BETA_OMEGA;
     }
     s.precision = _settings->hipPrecision;
     s.step_mode = _settings->hipStepMode;

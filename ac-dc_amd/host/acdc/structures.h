@@ -70,6 +70,9 @@ class OptimizationSettings {
   int hipLaunchMode = ACDC_LAUNCH_AUTO;
   int hipStepMode = ACDC_STEP_SYNC;           // ACDC_STEP_ASYNC: the multicore executor's
                                               // unsynchronised workers (tau = totalThreads)
+  bool hipSpectralBeta = false;               // beta from the largest eigenvalue of the
+                                              // normalised Gram matrix (mSDCA.cpp:188-286)
+                                              // unless forcedSigma / instance.sigma is set
 };
 
 class OptimizationStatistics {

@@ -50,8 +50,13 @@ typedef enum acdc_beta_mode {
                            per row: MultiCoreSolver.cpp:32-38, :108-110 */
   ACDC_BETA_FORCED = 1, /* beta = forced_beta (OptimizationSettings
                            ::forcedSigma, settingsAndStatistics.h:53) */
-  ACDC_BETA_NONE = 2    /* beta = 1, the multicore Lasso's L_j = 1/||a_j||^2
+  ACDC_BETA_NONE = 2,   /* beta = 1, the multicore Lasso's L_j = 1/||a_j||^2
                            (MulticoreSquareLoss.h:27-40) */
+  ACDC_BETA_SPECTRAL = 3 /* beta = 1+(tau-1)(lmax-1)/(n-1), lmax the largest
+                            eigenvalue of D^-1/2 A^T A D^-1/2 (segments as
+                            columns, D = diag ||a_c||^2) by 20 power steps
+                            from a start seeded by `seed`: mSDCA.cpp:188-265,
+                            :286 */
 } acdc_beta_mode;
 
 typedef enum acdc_step_mode {

@@ -27,6 +27,10 @@
  *                       orc_sample_steps gives it (bit for bit).
  *   orc_cd_async_omp    1 thread == orc_cd_list over orc_async_coords, bit
  *                       for bit; many threads converge to the serial optimum.
+ *   orc_spectral_lmax   closed forms: orthonormal segments give 1, n equal
+ *                       segments give n, two disjoint groups of p and q equal
+ *                       segments give max(p, q); numpy eigvalsh on a random
+ *                       matrix with a separated top eigenvalue.
  *
  * Compiled with -ffp-contract=off so that every a*b+c rounds twice, like the
  * reference's plain C++ loops.
@@ -179,6 +183,59 @@ int32_t orc_omega(int64_t nidx, int64_t nnz, const int32_t *idx) {
 double orc_eso_beta(int32_t omega, int64_t tau, int64_t n) {
   if (n <= 1) return 1.0;
   return 1.0 + (double)(omega - 1) * (double)(tau - 1) / ((double)n - 1.0);
+}
+
+/* start vector of the spectral power steps: coordinate c takes the first
+ * word of Philox4x32-10 at counter (c, c >> 32, "PCPM", 0) under the seed,
+ * over 2^32 */
+#define ORC_PM_TAG 0x5043504Du
+double orc_pm_start(uint64_t seed, int64_t c) {
+  const uint32_t ctr[4] = {(uint32_t)c, (uint32_t)((uint64_t)c >> 32), ORC_PM_TAG, 0u};
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t o[4];
+  orc_philox4x32_10(ctr, key, o);
+  return (double)o[0] * (1.0 / 4294967296.0);
+}
+
+/* Largest eigenvalue of D^-1/2 A^T A D^-1/2 (A's segments as its columns,
+ * D = diag ||a_c||^2) by power steps, as mSDCA.cpp:188-265: x normalised,
+ * then per step y = sum_c a_c x_c / ||a_c||, x_c = <a_c, y> / ||a_c||,
+ * estimate ||x||, x /= estimate.  Empty segments take 0. */
+double orc_spectral_lmax(int64_t nseg, int64_t nidx, const int32_t *ptr, const int32_t *idx,
+                         const double *vals, uint64_t seed, int iters) {
+  double *x = (double *)malloc(sizeof(double) * (size_t)(nseg > 0 ? nseg : 1));
+  double *w = (double *)malloc(sizeof(double) * (size_t)(nseg > 0 ? nseg : 1));
+  double *y = (double *)malloc(sizeof(double) * (size_t)(nidx > 0 ? nidx : 1));
+  double nrm = 0;
+  for (int64_t c = 0; c < nseg; ++c) {
+    double s2 = 0;
+    for (int32_t p = ptr[c]; p < ptr[c + 1]; ++p) s2 += vals[p] * vals[p];
+    w[c] = s2 > 0 ? 1.0 / sqrt(s2) : 0.0;
+    x[c] = orc_pm_start(seed, c);
+    nrm += x[c] * x[c];
+  }
+  nrm = sqrt(nrm);
+  for (int64_t c = 0; c < nseg; ++c) x[c] /= nrm;
+  double est = 1.0;
+  for (int it = 0; it < iters; ++it) {
+    for (int64_t k = 0; k < nidx; ++k) y[k] = 0;
+    for (int64_t c = 0; c < nseg; ++c)
+      for (int32_t p = ptr[c]; p < ptr[c + 1]; ++p) y[idx[p]] += vals[p] * w[c] * x[c];
+    double s2 = 0;
+    for (int64_t c = 0; c < nseg; ++c) {
+      double g = 0;
+      for (int32_t p = ptr[c]; p < ptr[c + 1]; ++p) g += vals[p] * y[idx[p]];
+      x[c] = g * w[c];
+      s2 += x[c] * x[c];
+    }
+    est = sqrt(s2);
+    if (est > 0)
+      for (int64_t c = 0; c < nseg; ++c) x[c] /= est;
+  }
+  free(x);
+  free(w);
+  free(y);
+  return est;
 }
 
 /* one coordinate's delta from the current state (compute_update) */

@@ -51,6 +51,13 @@ int32_t orc_omega(int64_t nidx, int64_t nnz, const int32_t *idx);
 /* ESO beta = 1 + (omega-1)(tau-1)/(n-1): MultiCoreSolver.cpp:108-110 */
 double orc_eso_beta(int32_t omega, int64_t tau, int64_t n);
 
+/* the spectral ESO (mSDCA.cpp:188-265, :286): beta = 1 + (tau-1)(lmax-1)/(n-1)
+ * with lmax the largest eigenvalue of D^-1/2 A^T A D^-1/2 by `iters` power
+ * steps from the seeded start orc_pm_start */
+double orc_pm_start(uint64_t seed, int64_t c);
+double orc_spectral_lmax(int64_t nseg, int64_t nidx, const int32_t *ptr, const int32_t *idx,
+                         const double *vals, uint64_t seed, int iters);
+
 /* nsteps synchronous tau-nice steps (cdn_common.h:235-282 semantics: all
  * deltas of a step from the same state, then all applied).  tau=1 is the
  * serial method do_single_iteration_serial, loss_square.h:72-83 with

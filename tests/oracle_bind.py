@@ -37,6 +37,8 @@ def lib() -> ctypes.CDLL:
             "orc_csc_lipschitz": ([_i64, _I32P, _F64P, _f64, _F64P], None),
             "orc_omega": ([_i64, _i64, _I32P], ctypes.c_int32),
             "orc_eso_beta": ([ctypes.c_int32, _i64, _i64], _f64),
+            "orc_pm_start": ([_u64, _i64], _f64),
+            "orc_spectral_lmax": ([_i64, _i64, _I32P, _I32P, _F64P, _u64, ctypes.c_int], _f64),
             "orc_cd_run": ([ctypes.c_int, _i64, _i64, _I32P, _I32P, _F64P, _F64P, _f64, _u64, _i64,
                             _u64, _u64, _F64P, _F64P], None),
             "orc_lasso_eval": ([_i64, _i64, _I32P, _I32P, _F64P, _F64P, _F64P, _f64, _F64P], None),
@@ -113,6 +115,18 @@ def omega(nidx, idx) -> int:
 
 def eso_beta(om, tau, n) -> float:
     return lib().orc_eso_beta(om, tau, n)
+
+
+def pm_start(seed, c) -> float:
+    return lib().orc_pm_start(seed, c)
+
+
+def spectral_lmax(nseg, nidx, ptr, idx, vals, seed, iters=20) -> float:
+    """Largest eigenvalue of D^-1/2 A^T A D^-1/2 by power steps (segments as
+    columns)."""
+    ptr, idx, vals = i32(ptr), i32(idx), f64(vals)
+    return lib().orc_spectral_lmax(nseg, nidx, _p(ptr, ctypes.c_int32), _p(idx, ctypes.c_int32),
+                                   _p(vals, ctypes.c_double), seed, iters)
 
 
 def csc_residual(m, n, cp, ri, v, x, b) -> np.ndarray:

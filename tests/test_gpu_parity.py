@@ -186,6 +186,39 @@ def test_nnls_ttd_matches_oracle(acdc, orc):
         s.close()
 
 
+def test_spectral_beta_matches_oracle(acdc, orc, c1):
+    """ACDC_BETA_SPECTRAL: the device's 20 power steps give the oracle's
+    eigenvalue (same seeded start), for CSC columns (Lasso) and CSR sample
+    rows (SVM); n equal segments give lmax = n, so beta = tau."""
+    p = c1.problem()
+    for tau in (1, 512):
+        s = acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=c1.lam, tau=tau, seed=21,
+                        beta_mode=acdc.BETA_SPECTRAL)
+        lmax = orc.spectral_lmax(c1.n, c1.m, c1.ptr, c1.idx, c1.vals, seed=21)
+        want = 1.0 + (tau - 1) * (lmax - 1.0) / (c1.n - 1.0)
+        assert s.beta == pytest.approx(want, rel=1e-10)
+        assert 1.0 <= s.beta <= orc.eso_beta(orc.omega(c1.m, c1.idx), tau, c1.n) + 1e-12
+        s.close()
+    d = acdc.gen_svm(2000, 50_000, row_median=300.0, row_sigma=0.901, wstar_nnz=1000,
+                     flip_frac=0.1, seed=3)
+    s = acdc.Solver(d.problem(), acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=256, seed=22,
+                    beta_mode=acdc.BETA_SPECTRAL)
+    lmax = orc.spectral_lmax(d.n, d.m, d.ptr, d.idx, d.vals, seed=22)
+    assert s.beta == pytest.approx(1.0 + 255 * (lmax - 1.0) / (d.n - 1.0), rel=1e-10)
+    s.close()
+    # 64 columns, each a multiple of one vector: the normalised Gram matrix is all ones
+    m, n = 50, 64
+    rng = np.random.default_rng(4)
+    base = rng.normal(size=m)
+    A = base[:, None] * rng.uniform(0.5, 2.0, size=n)[None, :]
+    cp = np.arange(n + 1) * m
+    ri = np.tile(np.arange(m), n)
+    q = acdc.Problem.from_csc(m, n, cp, ri, A.T.ravel(), rng.normal(size=m))
+    s = acdc.Solver(q, acdc.LOSS_L1_SQUARE, lam=0.1, tau=16, seed=1, beta_mode=acdc.BETA_SPECTRAL)
+    assert s.beta == pytest.approx(16.0, rel=1e-12)
+    s.close()
+
+
 def test_svm_matches_oracle(acdc, orc):
     d = acdc.gen_svm(2000, 50_000, row_median=300.0, row_sigma=0.901, wstar_nnz=1000,
                      flip_frac=0.1, seed=3)

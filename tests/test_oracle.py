@@ -88,6 +88,71 @@ def test_omega_and_beta(orc):
     assert orc.eso_beta(304, 4096, 10**6) == pytest.approx(1 + 303 * 4095 / 999999)
 
 
+def _csc(cols, m):
+    """CSC arrays of a list of dense columns (zeros dropped)."""
+    ptr, idx, val = [0], [], []
+    for c in cols:
+        nz = np.nonzero(c)[0]
+        idx += nz.tolist()
+        val += np.asarray(c)[nz].tolist()
+        ptr.append(len(idx))
+    return np.array(ptr), np.array(idx), np.array(val, dtype=float)
+
+
+def test_spectral_lmax_closed_forms(orc):
+    rng = np.random.default_rng(7)
+    m = 40
+    # orthogonal segments (disjoint supports): the normalised Gram matrix is I
+    cols = []
+    for k in range(10):
+        c = np.zeros(m)
+        c[4 * k:4 * k + 4] = rng.normal(size=4)
+        cols.append(c)
+    ptr, idx, val = _csc(cols, m)
+    assert orc.spectral_lmax(10, m, ptr, idx, val, seed=3) == pytest.approx(1.0, rel=1e-12)
+    # n equal segments (any scales): the normalised Gram matrix is all ones
+    base = rng.normal(size=m)
+    cols = [base * s for s in rng.uniform(0.5, 2.0, size=12)]
+    ptr, idx, val = _csc(cols, m)
+    assert orc.spectral_lmax(12, m, ptr, idx, val, seed=3) == pytest.approx(12.0, rel=1e-12)
+    # two disjoint groups of 5 and 3 equal segments: eigenvalues 5 and 3 (and 0)
+    a, b = np.zeros(m), np.zeros(m)
+    a[:20] = rng.normal(size=20)
+    b[20:] = rng.normal(size=20)
+    cols = [a * 1.5] * 5 + [b * 0.7] * 3
+    ptr, idx, val = _csc(cols, m)
+    assert orc.spectral_lmax(8, m, ptr, idx, val, seed=3) == pytest.approx(5.0, rel=1e-8)
+    # an empty segment is left out of the operator
+    ptr2 = np.concatenate([ptr, [ptr[-1]]])
+    assert orc.spectral_lmax(9, m, ptr2, idx, val, seed=3) == pytest.approx(5.0, rel=1e-8)
+
+
+def test_spectral_lmax_matches_eigvalsh(orc):
+    """A random sparse matrix plus a shared direction, so the top eigenvalue
+    of D^-1/2 A^T A D^-1/2 stands well clear of the rest and 20 power steps
+    reach it."""
+    rng = np.random.default_rng(11)
+    m, n = 60, 30
+    shared = rng.normal(size=m)
+    A = rng.normal(size=(m, n)) * (rng.uniform(size=(m, n)) < 0.2) + 0.8 * shared[:, None]
+    ptr, idx, val = _csc([A[:, j] for j in range(n)], m)
+    d = 1.0 / np.linalg.norm(A, axis=0)
+    M = (A * d).T @ (A * d)
+    ev = np.linalg.eigvalsh(M)
+    assert ev[-1] > 3 * ev[-2]
+    assert orc.spectral_lmax(n, m, ptr, idx, val, seed=5) == pytest.approx(ev[-1], rel=1e-9)
+    # and it lies in [1, omega]
+    om = np.count_nonzero(A, axis=1).max()
+    assert 1.0 <= ev[-1] <= om
+
+
+def test_pm_start_is_seeded_uniform(orc):
+    u = np.array([orc.pm_start(9, c) for c in range(4000)])
+    assert ((u >= 0) & (u < 1)).all()
+    assert abs(u.mean() - 0.5) < 0.02
+    assert orc.pm_start(9, 17) != orc.pm_start(10, 17)
+
+
 # -------------------------------------------------------------- square loss
 def test_one_column_closed_form(orc):
     rng = np.random.default_rng(2)
