@@ -141,6 +141,8 @@ class HipEngineExecutor : public AbstractEngineExecutor<L, D> {
     instance.primalObjective = (D)last_.primal;
     instance.dualObjective = (D)last_.dual;
     instance.oneZeroAccuracy = (D)last_.accuracy;
+    if (this->settings->lossFunction == ACDC_LOSS_SVM_DUAL_HINGE && !instance.test_b.empty())
+      test_accuracy_ = test_accuracy();
     return (D)(this->settings->lossFunction == ACDC_LOSS_SVM_DUAL_HINGE ? last_.gap : last_.primal);
   }
 
@@ -148,13 +150,33 @@ class HipEngineExecutor : public AbstractEngineExecutor<L, D> {
     return acdc_losses::log_headers(this->settings->lossFunction);
   }
   std::string getResultLogRow(OptimizationStatistics &statistics) override {
-    return acdc_losses::log_row(this->settings->lossFunction, last_, statistics);
+    return acdc_losses::log_row(this->settings->lossFunction, last_, statistics, test_accuracy_);
   }
 
   const acdc_report &lastReport() const { return last_; }
   acdc_solver *handle() { return solver_; }
 
  private:
+  // the test-set 0-1 accuracy of w, on the host as in
+  // MulticoreSVMDualLoss.h:93-110: a test row counts when y <x, w> > 0
+  double test_accuracy() {
+    std::vector<double> w((size_t)instance.m);
+    acdc_host::check(acdc_get_residuals(solver_, w.data(), (int64_t)w.size()),
+                     "acdc_get_residuals");
+    const size_t rows = instance.A_test_csr_row_ptr.empty() ? 0 : instance.A_test_csr_row_ptr.size() - 1;
+    if (rows == 0) return 0.0;
+    size_t good = 0;
+    for (size_t j = 0; j < rows; ++j) {
+      double e = 0;
+      for (L i = instance.A_test_csr_row_ptr[j]; i < instance.A_test_csr_row_ptr[j + 1]; ++i) {
+        const L f = instance.A_test_csr_col_idx[i];
+        if (f >= 0 && (size_t)f < w.size()) e += (double)instance.A_test_csr_values[i] * w[(size_t)f];
+      }
+      if ((double)instance.test_b[j] * e > 0) ++good;
+    }
+    return (double)good / (double)rows;
+  }
+
   void pull_x() {
     std::vector<double> x((size_t)instance.n);
     acdc_host::check(acdc_get_x(solver_, x.data(), (int64_t)x.size()), "acdc_get_x");
@@ -164,4 +186,5 @@ class HipEngineExecutor : public AbstractEngineExecutor<L, D> {
   acdc_problem *problem_ = nullptr;
   acdc_solver *solver_ = nullptr;
   acdc_report last_{};
+  double test_accuracy_ = 0.0;
 };

@@ -26,13 +26,15 @@ inline std::string log_headers(int loss) {
          "Duality gap";
 }
 
-inline std::string log_row(int loss, const acdc_report &r, const OptimizationStatistics &st) {
+// test_accuracy: the SVM's test-set 0-1 accuracy (0 without a test set, as
+// in MulticoreSVMDualLoss.h:93-110)
+inline std::string log_row(int loss, const acdc_report &r, const OptimizationStatistics &st,
+                           double test_accuracy = 0.0) {
   std::stringstream ss;
   ss << st.elapsedIterations << "," << std::setprecision(16);
   if (loss == ACDC_LOSS_SVM_DUAL_HINGE) {
-    // the test-set 0-1 accuracy needs a test set on the device: not part of this path
     ss << r.gap << "," << r.primal << "," << r.dual << "," << r.accuracy << ","
-       << st.elapsedPureComputationTime << "," << st.elapsed_time << "," << 0;
+       << st.elapsedPureComputationTime << "," << st.elapsed_time << "," << test_accuracy;
   } else {
     ss << r.primal << "," << st.elapsedPureComputationTime << "," << st.elapsed_time << ","
        << (loss == ACDC_LOSS_NNLS ? r.kkt : r.gap);

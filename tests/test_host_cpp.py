@@ -55,3 +55,19 @@ def test_cli_lasso_matches_oracle(acdc, orc, tmp_path):
     assert last == pytest.approx(o.eval()["primal"], rel=1e-10)
     rows = open(tmp_path / "log.csv").read().strip().splitlines()
     assert len(rows) == bulks + 1 and rows[-1].startswith(f"{bulks},")
+
+
+@pytest.mark.gpu
+def test_svm_executor_test_accuracy(acdc, tmp_path):
+    """The SVM log row's test-set 0-1 column (computed on the host from the
+    device's w) equals the device's training 0-1 accuracy when the test set
+    is the training set."""
+    exe = tmp_path / "test_svm_test_accuracy"
+    lib = os.path.join(ROOT, "ac-dc_amd", "lib")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror",
+                    "-I", os.path.join(ROOT, "ac-dc_amd", "host"),
+                    os.path.join(ROOT, "tests", "cpp", "test_svm_test_accuracy.cpp"),
+                    "-L", lib, "-lacdc_hip", f"-Wl,-rpath,{lib}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok")
