@@ -13,7 +13,8 @@
 //   -r <file>         CSV log file
 // and the HIP knobs (no reference counterpart):
 //   -s <seed>  -p <0 fp64 | 1 fp32+refresh>  -R <refresh epochs>  -L <lanes>
-//   -P <0 auto | 1 per-step launches | 2 persistent grid>
+//   -P <0 auto | 1 per-step launches | 2 persistent grid>  -M <0 sync | 1 async>
+//   -S <0|1>   spectral ESO beta (the power method of mSDCA.cpp:188-286)
 // In the reference, -I falls through into -M (option_console_parser.h:89-93);
 // here each flag sets only its own field.  omega and beta are computed as in
 // MultiCoreSolver.cpp:24-38, :55-62, :108-110.
@@ -33,7 +34,8 @@ static void usage() {
   std::cerr << "usage: acdc_hip_solver -A <data> [-b <b file>] -l <lambda> [-t tau] [-f loss]\n"
                "                       [-I updates per bulk] [-c bulks] [-i 0|1] [-r log]\n"
                "                       [-s seed] [-p precision] [-R refresh epochs] [-L lanes]\n"
-               "                       [-P launch mode] [-M step mode: 0 sync, 1 async]\n";
+               "                       [-P launch mode] [-M step mode: 0 sync, 1 async]\n"
+               "                       [-S 1: spectral ESO beta]\n";
 }
 
 int main(int argc, char *argv[]) {
@@ -42,7 +44,7 @@ int main(int argc, char *argv[]) {
   std::string matrixA, vectorB, logPath;
   double lambda = 0;
   int c;
-  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:s:p:R:L:P:M:h")) != -1) {
+  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:s:p:R:L:P:M:S:h")) != -1) {
     switch (c) {
       case 'A': matrixA = optarg; break;
       case 'b': vectorB = optarg; break;
@@ -65,6 +67,7 @@ int main(int argc, char *argv[]) {
       case 'L': settings.hipLanesPerCoord = std::atoi(optarg); break;
       case 'P': settings.hipLaunchMode = std::atoi(optarg); break;
       case 'M': settings.hipStepMode = std::atoi(optarg); break;
+      case 'S': settings.hipSpectralBeta = std::atoi(optarg) != 0; break;
       default: usage(); return c == 'h' ? 0 : 2;
     }
   }

@@ -5,6 +5,11 @@
 //   whitespace-separated; n = |colPtr| - 1, m = |b|.  Unlike the reference's
 //   loadVectorData (:33-48), a final value without a trailing newline is kept,
 //   and a missing file is an error rather than an empty vector.
+// Text + binary CSC of one rank (loadANSCIAndBinaryData, file_reader.h:50-106):
+//   <prefix>_colptr_<rank><suffix> holds n+1 whitespace-separated column
+//   pointers as text, <prefix>_rowid_<rank><suffix> the row indices as raw
+//   L, <prefix>_values_<rank><suffix> the values as raw D (host byte order);
+//   b from a text file.  Short or missing files are errors.
 // LIBSVM rows (loadDistributedSparseSVMRowData,
 //   cpp/src/utils/distributed_instances_loader.h:68-193): "label idx:val ..."
 //   per sample, 1-based feature indices shifted to 0-based, label 0 read as -1;
@@ -41,6 +46,36 @@ void loadCSCData(const std::string &matrixPrefix, const std::string &vectorBFile
   loadVectorData(matrixPrefix + "_rowIdx.txt", instance.A_csc_row_idx);
   if (instance.A_csc_col_ptr.empty()) throw std::runtime_error("empty column pointer file");
   instance.n = (L)(instance.A_csc_col_ptr.size() - 1);
+  instance.m = (L)instance.b.size();
+  instance.lambda = (D)lambda;
+}
+
+template <typename L, typename D>
+void loadAsciiAndBinaryCSC(const std::string &matrixPrefix, const std::string &matrixSuffix,
+                           const std::string &vectorBFileName, double lambda, L local_n,
+                           int rank, ProblemData<L, D> &instance) {
+  loadVectorData(vectorBFileName, instance.b);
+  const std::string tag = "_" + std::to_string(rank) + matrixSuffix;
+  {
+    const std::string path = matrixPrefix + "_colptr" + tag;
+    std::ifstream f(path.c_str());
+    if (!f.is_open()) throw std::runtime_error("file not found: " + path);
+    instance.A_csc_col_ptr.resize((size_t)local_n + 1);
+    for (auto &e : instance.A_csc_col_ptr)
+      if (!(f >> e)) throw std::runtime_error("short column pointer file: " + path);
+  }
+  const size_t nnz = (size_t)instance.A_csc_col_ptr[(size_t)local_n];
+  auto raw = [&](const std::string &name, auto &v) {
+    const std::string path = matrixPrefix + name + tag;
+    std::ifstream f(path.c_str(), std::ios::in | std::ios::binary);
+    if (!f.is_open()) throw std::runtime_error("file not found: " + path);
+    v.resize(nnz);
+    f.read(reinterpret_cast<char *>(v.data()), (std::streamsize)(nnz * sizeof(v[0])));
+    if ((size_t)f.gcount() != nnz * sizeof(v[0])) throw std::runtime_error("short binary file: " + path);
+  };
+  raw("_rowid", instance.A_csc_row_idx);
+  raw("_values", instance.A_csc_values);
+  instance.n = local_n;
   instance.m = (L)instance.b.size();
   instance.lambda = (D)lambda;
 }

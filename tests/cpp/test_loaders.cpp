@@ -40,6 +40,31 @@ int main(int argc, char **argv) {
     }
     CHECK(threw);
   }
+  {  // text column pointers + raw row ids and values of one rank
+    const int rows[3] = {0, 2, 1};
+    const double vals[3] = {1.5, -2.0, 0.25};
+    std::ofstream(dir + "/m_colptr_3.bin") << "0 2 3\n";
+    std::ofstream(dir + "/m_rowid_3.bin", std::ios::binary)
+        .write(reinterpret_cast<const char *>(rows), sizeof(rows));
+    std::ofstream(dir + "/m_values_3.bin", std::ios::binary)
+        .write(reinterpret_cast<const char *>(vals), sizeof(vals));
+    std::ofstream(dir + "/mb.txt") << "1\n2\n3\n";
+    ProblemData<int, double> p;
+    InputOuputHelper::loadAsciiAndBinaryCSC(dir + "/m", std::string(".bin"), dir + "/mb.txt", 0.5,
+                                            2, 3, p);
+    CHECK(p.m == 3 && p.n == 2 && p.lambda == 0.5);
+    CHECK(p.A_csc_col_ptr[2] == 3 && p.A_csc_row_idx[1] == 2 && p.A_csc_values[2] == 0.25);
+    bool threw = false;
+    try {  // a short values file
+      std::ofstream(dir + "/m_values_3.bin", std::ios::binary)
+          .write(reinterpret_cast<const char *>(vals), sizeof(double));
+      InputOuputHelper::loadAsciiAndBinaryCSC(dir + "/m", std::string(".bin"), dir + "/mb.txt",
+                                              0.5, 2, 3, p);
+    } catch (const std::exception &) {
+      threw = true;
+    }
+    CHECK(threw);
+  }
   {  // LIBSVM: 1-based features, label 0 -> -1
     std::ofstream(dir + "/svm.txt") << "+1 1:0.5 3:2\n0 2:1\n-1 4:-1.5 1:3\n";
     ProblemData<int, double> p;
@@ -62,6 +87,7 @@ int main(int argc, char **argv) {
     st.elapsed_time = 2.0;
     CHECK(acdc_losses::log_row(ACDC_LOSS_L1_SQUARE, r, st) == "7,2.5,1,2,0.5");
     CHECK(acdc_losses::log_row(ACDC_LOSS_SVM_DUAL_HINGE, r, st) == "7,0.5,2.5,2,0.75,1,2,0");
+    CHECK(acdc_losses::log_row(ACDC_LOSS_SVM_DUAL_HINGE, r, st, 0.5) == "7,0.5,2.5,2,0.75,1,2,0.5");
     CHECK(acdc_losses::log_headers(ACDC_LOSS_SVM_DUAL_HINGE).rfind("Elapsed iterations, Current duality gap", 0) == 0);
   }
   std::printf("host mirror loaders: ok\n");
