@@ -156,6 +156,7 @@ acdc_status enqueue_run(acdc_batch *b) {
   a.max_epochs = b->set.max_epochs;
   a.gap_tol = b->set.gap_tol;
   a.queue = b->dqueue;
+  a.chunk = std::max(1, b->set.queue_chunk);
   a.results = b->dres;
   a.lds_doubles = (int32_t)(kLdsBytes / 8);
   void *args[] = {&a};
@@ -235,6 +236,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   // tau per instance: the block's coordinate slots per synchronous step, or
   // its lane groups as asynchronous workers
   const int32_t cap = (b->async ? 1 : kSlotsPerGroup) * (kBatchBlock / b->lanes);
+  if (set->queue_chunk < 0)
+    return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: queue_chunk must be >= 0"));
   if (set->tau < 0 || set->tau > cap)
     return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: tau must lie in [0, " +
                                             std::to_string(cap) + "] for these lanes and mode"));

@@ -78,6 +78,7 @@ struct BatchArgs {
   int32_t max_epochs;
   double gap_tol;
   unsigned int *queue;
+  int32_t chunk;         // queue items a block takes per queue access
   BatchResult *results;  // instances x path_len
   int32_t lds_doubles;   // FP64 rows the dynamic LDS holds (refresh residual)
 };
@@ -527,14 +528,19 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
   __shared__ double sh[kRefreshStats * kBatchWaves];
   __shared__ BatchResult s_res;
   __shared__ int s_item;
+  int next = 0, end = 0;  // the block's current chunk of queue items
   for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.queue, 1u);
-    __syncthreads();
-    // readfirstlane marks the block-uniform item as such, so the instance's
-    // fields and pointers live in scalar registers
-    const int item = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
-    if (item >= a.count) break;
+    if (next >= end) {
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(a.queue, (unsigned)a.chunk);
+      __syncthreads();
+      // readfirstlane marks the block-uniform item as such, so the
+      // instance's fields and pointers live in scalar registers
+      next = __builtin_amdgcn_readfirstlane(s_item);
+      end = min(next + a.chunk, a.count);
+      __syncthreads();
+      if (next >= a.count) break;
+    }
+    const int item = next++;
     const int i = a.order[item];
     const BatchInstance *Ip = a.inst + i;
     const BatchInstance I = *Ip;
@@ -714,12 +720,17 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
   __shared__ double sh[kRefreshStats * kBatchWaves];
   __shared__ BatchResult s_res;
   __shared__ int s_item;
+  int next = 0, end = 0;  // the block's current chunk of queue items
   for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.queue, 1u);
-    __syncthreads();
-    const int item = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
-    if (item >= a.count) break;
+    if (next >= end) {
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(a.queue, (unsigned)a.chunk);
+      __syncthreads();
+      next = __builtin_amdgcn_readfirstlane(s_item);
+      end = min(next + a.chunk, a.count);
+      __syncthreads();
+      if (next >= a.count) break;
+    }
+    const int item = next++;
     const int i = a.order[item];
     const BatchInstance *Ip = a.inst + i;
     const BatchInstance I = *Ip;

@@ -1,6 +1,7 @@
-// tune.hip -- the autotuner: picks lanes per coordinate and tau for the
-// batched path, and lanes per coordinate for a single solver, by timing short
-// runs on the device through the library's own entry points.
+// tune.hip -- the autotuner: picks lanes per coordinate, tau and queue
+// chunking for the batched path, and lanes per coordinate for a single
+// solver, by timing short runs on the device through the library's own
+// entry points.
 //
 // Score of a batch candidate: measured progress per second, log(gap(x = 0) /
 // gap after the trial) over the trial's device time, at one fixed lambda and
@@ -91,6 +92,41 @@ extern "C" acdc_status acdc_batch_tune(const acdc_problem *const *problems, int3
         best = sc;
         pick.lanes_per_coord = lanes;
         pick.tau = tau;
+      }
+    }
+  }
+  // queue chunking: one instance per queue access while the batch has no
+  // more instances than resident blocks (a larger chunk could only leave
+  // blocks idle); otherwise the chunk with the least device time over a
+  // short run of the whole batch (two lambdas, two epochs each)
+  pick.queue_chunk = 1;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      count > cus) {
+    std::vector<double> lams((size_t)count * 2);
+    for (int32_t i = 0; i < count; ++i) {
+      const double lm = lambda_max(problems[i]);
+      lams[2 * (size_t)i] = 0.5 * lm;
+      lams[2 * (size_t)i + 1] = 0.1 * lm;
+    }
+    double best_ms = -1;
+    for (int chunk : {1, 2, 4, 8}) {
+      acdc_batch_settings s = pick;
+      s.queue_chunk = chunk;
+      s.refresh_period = 2;
+      s.max_epochs = 2;
+      s.gap_tol = -1.0;
+      s.use_graph = 0;
+      acdc_batch *b = nullptr;
+      acdc_status st = acdc_batch_create(problems, count, lams.data(), 2, &s, &b);
+      double ms = 0;
+      if (st == ACDC_OK) st = acdc_batch_run(b, &ms);
+      if (b) acdc_batch_destroy(b);
+      if (st != ACDC_OK) return st;
+      if (best_ms < 0 || ms < best_ms) {
+        best_ms = ms;
+        pick.queue_chunk = chunk;
       }
     }
   }

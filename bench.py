@@ -163,11 +163,12 @@ def batched(acdc, args):
     probs = [d.problem() for d in ds]
     lams = np.stack([acdc.lambda_path(d.lambda_max, args.batch_path, 1e-3) for d in ds])
     tuned = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC)
-    log(f"[bench] batched: autotuned lanes {tuned['lanes_per_coord']}, tau {tuned['tau']}")
+    log(f"[bench] batched: autotuned lanes {tuned['lanes_per_coord']}, tau {tuned['tau']}, "
+        f"queue chunk {tuned['queue_chunk']}")
     kw = dict(seed=5, precision=acdc.PREC_FP32_REFRESH, refresh_period=5,
               max_epochs=args.batch_max_epochs, gap_tol=args.gap_target,
               step_mode=acdc.STEP_ASYNC, lanes_per_coord=tuned["lanes_per_coord"],
-              tau=tuned["tau"])
+              tau=tuned["tau"], queue_chunk=tuned["queue_chunk"])
     b = acdc.Batch(probs, lams, use_graph=True, **kw)  # uploads, captures the graph
     t0 = time.perf_counter()
     ms = b.run()

@@ -246,6 +246,9 @@ typedef struct acdc_batch_settings {
   int32_t tau;             /* 0 = the block's capacity (SYNC: 2 slots per lane
                               group; ASYNC: one worker per lane group), else
                               at most that, per instance at most n */
+  int32_t queue_chunk;     /* instances a block takes from the work queue per
+                              access (0 = 1); consecutive in the queue's
+                              largest-first order */
 } acdc_batch_settings;
 
 typedef struct acdc_batch_result {
@@ -274,8 +277,11 @@ acdc_status acdc_batch_destroy(acdc_batch *b);
  * epochs at lambda = 0.1 lambda_max) for lanes per coordinate in {8,16,32}
  * and tau in {cap/4, cap/2, cap} (cap: the block's slots or workers for
  * those lanes) and returns *base with the lanes and tau of the best score =
- * log(gap at x = 0 / gap after the trial) per second of device time.  The work queue hands out one instance at a
- * time (largest first), so there is no chunk size to tune.
+ * log(gap at x = 0 / gap after the trial) per second of device time, and
+ * with queue_chunk = 1 when the batch has no more instances than the device
+ * has CUs (a block per CU: a larger chunk would only idle blocks), else the
+ * chunk in {1, 2, 4, 8} with the least device time over the whole batch at
+ * two lambdas of two epochs each.
  * acdc_solver_tune_lanes times two epochs (after a warm-up epoch) at lanes
  * 16, 32 and 64 with the other settings of *base and returns the fastest. */
 acdc_status acdc_batch_tune(const acdc_problem *const *problems, int32_t count,

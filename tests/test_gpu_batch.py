@@ -41,13 +41,15 @@ def oracle_path(orc, d, lams, seed, tau, beta, epochs):
     return o
 
 
-@pytest.mark.parametrize("lanes", [8, 16, 32])
-def test_batch_fp64_fixed_epochs_matches_oracle(acdc, orc, small_batch, lanes):
+@pytest.mark.parametrize("lanes,chunk", [(8, 0), (16, 0), (32, 0), (16, 3)])
+def test_batch_fp64_fixed_epochs_matches_oracle(acdc, orc, small_batch, lanes, chunk):
+    """chunk 3: a block takes three instances per queue access"""
     ds = small_batch
     lams = lambdas_of(acdc, ds, 3, 0.1)
     probs = [d.problem() for d in ds]
     b = acdc.Batch(probs, lams, seed=40, precision=acdc.PREC_FP64, lanes_per_coord=lanes,
-                   refresh_period=1, max_epochs=2, gap_tol=-1.0, use_graph=False)
+                   refresh_period=1, max_epochs=2, gap_tol=-1.0, use_graph=False,
+                   queue_chunk=chunk)
     b.run()
     res = b.results()
     assert (res["epochs"] == 2).all()
@@ -160,6 +162,7 @@ def test_autotuner_picks_valid_settings(acdc, small_batch):
     t = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC)
     assert t["lanes_per_coord"] in (8, 16, 32)
     assert 1 <= t["tau"] <= 1024 // t["lanes_per_coord"]
+    assert t["queue_chunk"] == 1  # fewer instances than CUs
     assert t["score"] > 0
     lams = lambdas_of(acdc, small_batch, 2, 0.3)
     b = acdc.Batch(probs, lams, seed=3, precision=acdc.PREC_FP32_REFRESH, max_epochs=400,
