@@ -194,7 +194,7 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: lanes_per_coord must be 0, 8, 16, 32 or 64");
   for (int32_t i = 0; i < count; ++i) {
     if (!problems[i] || problems[i]->kind != 0)
-      return bfail(ACDC_ERR_INVALID, "acdc_batch_create: every problem must be column-major (Lasso)");
+      return bfail(ACDC_ERR_INVALID, "acdc_batch_create: every problem must be column-major (Lasso, NNLS)");
     for (int32_t l = 0; l < path_len; ++l)
       if (!(lambdas[(int64_t)i * path_len + l] >= 0))
         return bfail(ACDC_ERR_INVALID, "acdc_batch_create: lambdas must be >= 0");
@@ -238,6 +238,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   const int32_t cap = (b->async ? 1 : kSlotsPerGroup) * (kBatchBlock / b->lanes);
   if (set->queue_chunk < 0)
     return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: queue_chunk must be >= 0"));
+  if (set->loss != ACDC_LOSS_L1_SQUARE && set->loss != ACDC_LOSS_NNLS)
+    return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: loss must be L1-square or NNLS"));
   if (set->tau < 0 || set->tau > cap)
     return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: tau must lie in [0, " +
                                             std::to_string(cap) + "] for these lanes and mode"));
@@ -288,6 +290,13 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     I.ent = dent + onnz * esz;
     I.rent = drent + onnz;
     I.nnz = (int32_t)p->nnz;
+    I.nonneg = set->loss == ACDC_LOSS_NNLS;
+    I.scale = 0;
+    for (int64_t j = 0; j < p->ncoord; ++j) {  // ||A^T b||_inf
+      double g = 0;
+      for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) g += p->cval[q] * p->rhs[p->cidx[q]];
+      I.scale = std::max(I.scale, std::fabs(g));
+    }
     I.b = db + om;
     I.r64 = dr64 + om;
     // residual rows beyond the LDS capacity go to the instance's global part

@@ -58,8 +58,19 @@ struct BatchInstance {
   int32_t tau;           // coordinates per step of this instance
   int32_t lds_rows;      // rows [0, lds_rows) of the correction residual live in LDS
   int32_t nnz;
+  int32_t nonneg;        // 1: NNLS (x >= 0, no lambda), 0: Lasso
+  double scale;          // NNLS: ||A^T b||_inf, the KKT residual's scale
   uint64_t seed;
 };
+
+// The coordinate step of an instance: the Lasso's soft threshold
+// (MulticoreL1RegSquareLoss.h:41-51) or, for an NNLS instance, the projected
+// step onto x >= 0 (prox_delta<kNnls>); the scatter factor is the delta.
+__device__ __forceinline__ double batch_delta(bool nonneg, double g, double xj, double Lj,
+                                             const StepScalars &sc) {
+  double c;
+  return nonneg ? prox_delta<kNnls>(g, xj, Lj, 0.0, sc, &c) : prox_delta<kLasso>(g, xj, Lj, 0.0, sc, &c);
+}
 
 struct BatchResult {
   double lambda, primal, dual, rel_gap, nnz_x;
@@ -415,17 +426,31 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
     r2 += rk * rk;
     br += gld(I.b + k) * rk;
   }
+  if (I.nonneg) {  // NNLS: the KKT residual max_j |min(x_j, g_j)| takes max|g|'s slot
+    ginf = 0;
+    for (int j = tid; j < I.n; j += kBatchBlock)
+      ginf = fmax(ginf, fabs(fmin(gld_agent(&I.cs[j].x), gld(&I.cs[j].gref))));
+  }
   double st[kRefreshStats] = {ginf, x1, nz, r2, br};
   block_stats(st, sh);  // its barrier also ends every read of r
   dr_view<T, LDSR>(I, lds).zero(I.m);
   if (tid == 0) {
-    const double P = 0.5 * st[3] + lambda * st[1];
-    const double s = st[0] > lambda ? lambda / st[0] : 1.0;
-    const double D = -s * st[4] - 0.5 * s * s * st[3];
-    out->primal = P;
-    out->dual = D;
-    out->nnz_x = st[2];
-    out->rel_gap = P != 0 ? (P - D) / fabs(P) : 0.0;
+    if (I.nonneg) {
+      // objective 1/2 ||Ax - b||^2 (no dual is formed: 0); rel_gap is the
+      // KKT residual over ||A^T b||_inf
+      out->primal = 0.5 * st[3];
+      out->dual = 0.0;
+      out->nnz_x = st[2];
+      out->rel_gap = I.scale > 0 ? st[0] / I.scale : st[0];
+    } else {
+      const double P = 0.5 * st[3] + lambda * st[1];
+      const double s = st[0] > lambda ? lambda / st[0] : 1.0;
+      const double D = -s * st[4] - 0.5 * s * s * st[3];
+      out->primal = P;
+      out->dual = D;
+      out->nnz_x = st[2];
+      out->rel_gap = P != 0 ? (P - D) / fabs(P) : 0.0;
+    }
   }
   __syncthreads();  // gref, dr and *out are settled for the steps
 }
@@ -497,7 +522,8 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
       T acc = col[k].dot(ent, dr, lane);
       acc = group_sum<G>(acc);
       const double g = rec[k].gref + (double)acc;
-      const double d = prox_delta<kLasso>(g, rec[k].x, rec[k].L, 0.0, sc, &c[k]);
+      const double d = batch_delta(I.nonneg, g, rec[k].x, rec[k].L, sc);
+      c[k] = d;
       if (lane == 0) *gp(&I.cs[j[k]].x) = rec[k].x + d;
     }
     __syncthreads();
@@ -678,9 +704,38 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
         acc += e.val * gather(e.row);
       }
     acc = group_sum<G>(acc);
-    double c;
-    const double d = prox_delta<kLasso>(cur.r.gref + (double)acc, xj, cur.r.L, 0.0, sc, &c);
-    if (lane == 0 && d != 0.0) gadd_agent(&I.cs[cur.j].x, d);
+    const double g = cur.r.gref + (double)acc;
+    double d;
+    if (!I.nonneg) {
+      d = batch_delta(false, g, xj, cur.r.L, sc);
+      if (lane == 0 && d != 0.0) gadd_agent(&I.cs[cur.j].x, d);
+    } else {
+      // x >= 0 under races: the new value is computed from the x_j it
+      // replaces, by compare-and-swap, and the scatter takes the delta that
+      // was swapped in (one worker: the first swap succeeds)
+      d = 0.0;
+      if (lane == 0) {
+        unsigned long long *px = reinterpret_cast<unsigned long long *>(&I.cs[cur.j].x);
+        double old = xj;
+        for (;;) {
+          const double dd = batch_delta(true, g, old, cur.r.L, sc);
+          const double nw = old + dd;
+          if (nw == old) {
+            d = dd;
+            break;
+          }
+          const unsigned long long prev =
+              atomicCAS(px, __double_as_longlong(old), __double_as_longlong(nw));
+          if (prev == (unsigned long long)__double_as_longlong(old)) {
+            d = dd;
+            break;
+          }
+          old = __longlong_as_double((long long)prev);
+        }
+      }
+      d = __shfl(d, 0, G);
+    }
+    const double c = d;
     j2 = j1;
     d2 = d1;
     j1 = cur.j;

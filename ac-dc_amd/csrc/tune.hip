@@ -47,7 +47,7 @@ extern "C" acdc_status acdc_batch_tune(const acdc_problem *const *problems, int3
   int32_t big = 0;
   for (int32_t i = 0; i < count; ++i) {
     if (!problems[i] || problems[i]->kind != 0)
-      return tfail(ACDC_ERR_INVALID, "acdc_batch_tune: every problem must be column-major (Lasso)");
+      return tfail(ACDC_ERR_INVALID, "acdc_batch_tune: every problem must be column-major (Lasso, NNLS)");
     if (problems[i]->nnz > problems[big]->nnz) big = i;
   }
   const acdc_problem *p = problems[big];

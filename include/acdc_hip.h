@@ -249,6 +249,11 @@ typedef struct acdc_batch_settings {
   int32_t queue_chunk;     /* instances a block takes from the work queue per
                               access (0 = 1); consecutive in the queue's
                               largest-first order */
+  int32_t loss;            /* of every instance: ACDC_LOSS_L1_SQUARE (0) or
+                              ACDC_LOSS_NNLS (x >= 0, the lambdas are ignored;
+                              primal = 1/2 ||Ax-b||^2, dual = 0, rel_gap = the
+                              KKT residual max_j |min(x_j, g_j)| over
+                              ||A^T b||_inf) */
 } acdc_batch_settings;
 
 typedef struct acdc_batch_result {

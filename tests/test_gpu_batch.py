@@ -174,3 +174,81 @@ def test_autotuner_picks_valid_settings(acdc, small_batch):
     d = small_batch[1]
     s = acdc.solver_tune_lanes(d.problem(), acdc.LOSS_L1_SQUARE, lam=d.lam, tau=256)
     assert s["lanes_per_coord"] in (16, 32, 64) and s["updates_per_s"] > 0
+
+
+@pytest.fixture(scope="module")
+def ttd_batch(acdc):
+    # truss ground structures of uneven sizes (NNLS: x >= 0, no lambda)
+    return [acdc.gen_ttd(k) for k in (2, 3, 4, 6)]
+
+
+def nnls_scale(d):
+    g = np.zeros(d.n)
+    np.add.at(g, np.repeat(np.arange(d.n), np.diff(d.ptr)), d.vals * d.rhs[d.idx])
+    return np.max(np.abs(g))
+
+
+def test_batch_nnls_sync_matches_oracle(acdc, orc, ttd_batch):
+    """NNLS instances, synchronous FP64 steps at fixed epochs: the oracle's
+    projected coordinate steps (loss 10) with the same sampling, tau and
+    beta; the refresh's objective and relative KKT residual."""
+    ds = ttd_batch
+    probs = [d.problem() for d in ds]
+    lams = np.zeros((len(ds), 1))
+    epochs = 3
+    b = acdc.Batch(probs, lams, seed=50, precision=acdc.PREC_FP64, refresh_period=1,
+                   max_epochs=epochs, gap_tol=-1.0, use_graph=False, loss=acdc.LOSS_NNLS)
+    b.run()
+    res = b.results()
+    for i, d in enumerate(ds):
+        tau, beta, _ = b.info(i)
+        o = orc.CD(10, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, 0.0, beta=beta)
+        o.run(seed=50 + i, tau=tau, step0=0, nsteps=epochs * (-(-d.n // tau)))
+        x = b.x(i)
+        assert (x >= 0).all()
+        assert np.max(np.abs(x - o.x)) / max(1.0, np.max(np.abs(o.x))) < 1e-10
+        e = o.eval()
+        assert res[i, 0]["primal"] == pytest.approx(e["primal"], rel=1e-10)
+        assert res[i, 0]["rel_gap"] == pytest.approx(e["kkt"] / nnls_scale(d), rel=1e-8, abs=1e-14)
+    b.close()
+
+
+def test_batch_nnls_async_one_worker_is_serial(acdc, orc, ttd_batch):
+    ds = ttd_batch
+    probs = [d.problem() for d in ds]
+    b = acdc.Batch(probs, np.zeros((len(ds), 1)), seed=60, precision=acdc.PREC_FP64,
+                   refresh_period=1, max_epochs=2, gap_tol=-1.0, use_graph=False,
+                   step_mode=acdc.STEP_ASYNC, tau=1, loss=acdc.LOSS_NNLS)
+    b.run()
+    res = b.results()
+    for i, d in enumerate(ds):
+        o = orc.CD(10, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, 0.0)
+        o.run_list(orc.async_coords(60 + i, 0, 0, 2 * d.n, d.n))
+        x = b.x(i)
+        assert np.max(np.abs(x - o.x)) / max(1.0, np.max(np.abs(o.x))) < 1e-10
+        assert res[i, 0]["primal"] == pytest.approx(o.eval()["primal"], rel=1e-10)
+    b.close()
+
+
+def test_batch_nnls_async_workers_converge(acdc, orc, ttd_batch):
+    """Many FP32 workers with compare-and-swap steps: x stays >= 0 and every
+    instance takes its relative KKT residual from 1 (at x = 0) below 1e-4
+    (the truss systems are ill-conditioned: four orders within the epoch
+    cap), which the FP64 oracle's evaluation of the device's x confirms."""
+    ds = ttd_batch
+    probs = [d.problem() for d in ds]
+    b = acdc.Batch(probs, np.zeros((len(ds), 1)), seed=61, precision=acdc.PREC_FP32_REFRESH,
+                   refresh_period=5, max_epochs=3000, gap_tol=1e-4, use_graph=True,
+                   step_mode=acdc.STEP_ASYNC, loss=acdc.LOSS_NNLS)
+    b.run()
+    res = b.results()
+    assert res["converged"].all(), res["rel_gap"]
+    for i, d in enumerate(ds):
+        x = b.x(i)
+        assert (x >= 0).all()
+        o = orc.CD(10, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, 0.0)
+        e = o.eval(x)
+        assert e["kkt"] / nnls_scale(d) == pytest.approx(res[i, 0]["rel_gap"], rel=1e-6, abs=1e-12)
+        assert e["kkt"] / nnls_scale(d) <= 1e-4
+        assert res[i, 0]["primal"] == pytest.approx(e["primal"], rel=1e-9)
+    b.close()
