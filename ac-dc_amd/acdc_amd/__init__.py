@@ -395,8 +395,10 @@ class Solver:
 
 
 class Batch:
-    """Many Lasso instances, each along a warm-started lambda path, in one
-    persistent launch (acdc_batch_* of the C ABI)."""
+    """Many Lasso, NNLS (loss=LOSS_NNLS) or hinge-SVM dual
+    (loss=LOSS_SVM_DUAL_HINGE, problems from Problem.csr) instances, each along
+    a warm-started lambda path, in one persistent launch (acdc_batch_* of the
+    C ABI).  For SVM instances x(i) is alpha."""
 
     def __init__(self, problems, lambdas, *, seed=1, precision=PREC_FP32_REFRESH,
                  lanes_per_coord=0, refresh_period=5, max_epochs=200, gap_tol=1e-6,

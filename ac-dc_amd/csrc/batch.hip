@@ -1,5 +1,5 @@
-// batch.hip -- C ABI of the batched multi-instance Lasso solver
-// (acdc_batch_* in include/acdc_hip.h).
+// batch.hip -- C ABI of the batched multi-instance solver: Lasso, NNLS or
+// hinge-SVM dual instances (acdc_batch_* in include/acdc_hip.h).
 //
 // Layout: all instances' arrays are packed into one allocation per kind
 // (coordinate records, (row, value) entries, FP64 values, b, r64, global dr),
@@ -45,16 +45,13 @@ acdc_status bfail(acdc_status st, const std::string &msg) {
   } while (0)
 
 constexpr size_t kLdsBytes = 160 * 1024 - 1024;  // dynamic LDS per block (static scratch aside)
-// coordinates per lane group and step; with 4 register chunks per column
-// this keeps every instantiation within 128 VGPRs (1024-thread blocks)
-// without scratch spills
-constexpr int kSlotsPerGroup = 2;
 }  // namespace
 
 struct acdc_batch {
   acdc_batch_settings set{};
   bool fp32 = true;
   bool async = false;  // ACDC_STEP_ASYNC inside instances
+  bool svm = false;    // hinge-SVM dual instances (kernels of their own)
   int lanes = 16;
   int32_t count = 0, path_len = 0;
   std::vector<int64_t> n, m, xoff;
@@ -101,42 +98,9 @@ void destroy_batch(acdc_batch *b) {
   delete b;
 }
 
-// column entries cached in registers per lane: 4 chunks, 2 for 64-lane groups
-constexpr int regs_for(int lanes) { return lanes == 64 ? 2 : 4; }
-
-// entries of a column an asynchronous worker caches in registers per lane
-// (FP32: 8 chunks for 8 and 16 lanes, 4 for 32, 2 for 64; FP64: at most 4),
-// the most that stays spill-free in 128 VGPRs; longer columns take the
-// variant with a tail loop
-template <typename T>
-constexpr int async_regs_for(int lanes) {
-  return lanes == 64 ? 2 : (lanes == 32 || sizeof(T) == 8 ? 4 : 8);
-}
-
-template <typename T, int G>
-const void *batch_fn(bool async, bool tail) {
-  constexpr int RA = async_regs_for<T>(G);
-  if (async)
-    return tail ? reinterpret_cast<const void *>(&k_batch_async<T, G, RA, true>)
-                : reinterpret_cast<const void *>(&k_batch_async<T, G, RA, false>);
-  return reinterpret_cast<const void *>(&k_batch_lasso<T, G, kSlotsPerGroup, regs_for(G)>);
-}
-
-template <typename T>
-const void *batch_fn_lanes(int lanes, bool async, bool tail) {
-  switch (lanes) {
-    case 8: return batch_fn<T, 8>(async, tail);
-    case 16: return batch_fn<T, 16>(async, tail);
-    case 32: return batch_fn<T, 32>(async, tail);
-    default: return batch_fn<T, 64>(async, tail);
-  }
-}
-
 const void *batch_kernel(const acdc_batch *b) {
-  const int ra = b->fp32 ? async_regs_for<float>(b->lanes) : async_regs_for<double>(b->lanes);
-  const bool tail = b->maxlen > (int64_t)ra * b->lanes;
-  return b->fp32 ? batch_fn_lanes<float>(b->lanes, b->async, tail)
-                 : batch_fn_lanes<double>(b->lanes, b->async, tail);
+  return b->svm ? batch_svm_kernel(b->fp32, b->lanes, b->async, b->maxlen)
+                : batch_kernel_for<false>(b->fp32, b->lanes, b->async, b->maxlen);
 }
 
 // enqueue the reset and the persistent launch on b->stream
@@ -192,12 +156,20 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   if (set->lanes_per_coord != 0 && set->lanes_per_coord != 8 && set->lanes_per_coord != 16 &&
       set->lanes_per_coord != 32 && set->lanes_per_coord != 64)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: lanes_per_coord must be 0, 8, 16, 32 or 64");
+  if (set->loss != ACDC_LOSS_L1_SQUARE && set->loss != ACDC_LOSS_NNLS &&
+      set->loss != ACDC_LOSS_SVM_DUAL_HINGE)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: loss must be L1-square, NNLS or SVM dual");
+  const bool svm = set->loss == ACDC_LOSS_SVM_DUAL_HINGE;
   for (int32_t i = 0; i < count; ++i) {
-    if (!problems[i] || problems[i]->kind != 0)
-      return bfail(ACDC_ERR_INVALID, "acdc_batch_create: every problem must be column-major (Lasso, NNLS)");
-    for (int32_t l = 0; l < path_len; ++l)
-      if (!(lambdas[(int64_t)i * path_len + l] >= 0))
-        return bfail(ACDC_ERR_INVALID, "acdc_batch_create: lambdas must be >= 0");
+    if (!problems[i] || problems[i]->kind != (svm ? 1 : 0))
+      return bfail(ACDC_ERR_INVALID, svm ? "acdc_batch_create: SVM problems must be sample rows (CSR)"
+                                         : "acdc_batch_create: Lasso and NNLS problems must be column-major");
+    for (int32_t l = 0; l < path_len; ++l) {
+      const double lam = lambdas[(int64_t)i * path_len + l];
+      if (svm ? !(lam > 0) : !(lam >= 0))
+        return bfail(ACDC_ERR_INVALID, svm ? "acdc_batch_create: SVM lambdas must be > 0"
+                                           : "acdc_batch_create: lambdas must be >= 0");
+    }
   }
   int32_t ndev = 0;
   acdc_device_count(&ndev);
@@ -214,6 +186,7 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   b->set = *set;
   b->fp32 = set->precision == ACDC_PREC_FP32_REFRESH;
   b->async = set->step_mode == ACDC_STEP_ASYNC;
+  b->svm = svm;
   b->count = count;
   b->path_len = path_len;
   int64_t tot_nnz = 0, tot_n = 0, tot_m = 0;
@@ -238,8 +211,6 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   const int32_t cap = (b->async ? 1 : kSlotsPerGroup) * (kBatchBlock / b->lanes);
   if (set->queue_chunk < 0)
     return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: queue_chunk must be >= 0"));
-  if (set->loss != ACDC_LOSS_L1_SQUARE && set->loss != ACDC_LOSS_NNLS)
-    return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: loss must be L1-square or NNLS"));
   if (set->tau < 0 || set->tau > cap)
     return bail(bfail(ACDC_ERR_INVALID, "acdc_batch_create: tau must lie in [0, " +
                                             std::to_string(cap) + "] for these lanes and mode"));
@@ -290,14 +261,14 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     I.ent = dent + onnz * esz;
     I.rent = drent + onnz;
     I.nnz = (int32_t)p->nnz;
-    I.nonneg = set->loss == ACDC_LOSS_NNLS;
+    I.loss = set->loss;  // the LossId values are the ABI's
     I.scale = 0;
-    for (int64_t j = 0; j < p->ncoord; ++j) {  // ||A^T b||_inf
+    for (int64_t j = 0; !svm && j < p->ncoord; ++j) {  // ||A^T b||_inf
       double g = 0;
       for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) g += p->cval[q] * p->rhs[p->cidx[q]];
       I.scale = std::max(I.scale, std::fabs(g));
     }
-    I.b = db + om;
+    I.b = svm ? nullptr : db + om;
     I.r64 = dr64 + om;
     // residual rows beyond the LDS capacity go to the instance's global part
     I.drg = nullptr;
@@ -318,13 +289,16 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     b->xoff.push_back(on);
     b->tau.push_back(I.tau);
     b->beta.push_back(beta);
-    // coordinate records: x = 0, L_j = 1/(beta ||a_j||^2) (loss_square.h:183-194)
+    // coordinate records: x = 0, L_j = 1/(beta ||a_j||^2) (loss_square.h:183-194;
+    // MulticoreSVMDualLoss.h:50-58 for samples), times the label y_j = +-1
+    // for SVM instances (the kernels read the label off L's sign)
     recs.resize(p->ncoord);
     for (int64_t j = 0; j < p->ncoord; ++j) {
       double s2 = 0;
       for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) s2 += p->cval[q] * p->cval[q];
       recs[j].x = 0.0;
       recs[j].L = s2 > 0 ? 1.0 / (beta * s2) : 0.0;
+      if (svm) recs[j].L *= p->rhs[j];
       recs[j].gref = 0.0;
       recs[j].p0 = p->cptr[j];
       recs[j].len = p->cptr[j + 1] - p->cptr[j];
@@ -350,7 +324,7 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
       e = hipMemcpy(const_cast<RefEntry *>(I.rent), re.data(), p->nnz * sizeof(RefEntry),
                     hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess)
+    if (e == hipSuccess && !svm)
       e = hipMemcpy(const_cast<double *>(I.b), p->rhs.data(), p->nres * 8, hipMemcpyHostToDevice);
     onnz += p->nnz;
     on += p->ncoord;

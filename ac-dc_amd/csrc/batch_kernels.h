@@ -1,5 +1,5 @@
-// batch_kernels.h -- one persistent launch that solves many Lasso instances,
-// each along a warm-started lambda path.
+// batch_kernels.h -- one persistent launch that solves many Lasso, NNLS or
+// hinge-SVM dual instances, each along a warm-started lambda path.
 //
 // Work distribution: a device work queue (one atomic counter) hands whole
 // instances to 1024-thread blocks, largest first.  Inside a block an instance
@@ -51,25 +51,65 @@ struct BatchInstance {
   CoordRec *cs;          // n records
   const void *ent;       // Entry<T>[nnz]
   const RefEntry *rent;  // FP64 entries of the refresh, entry order
-  const double *b;
+  const double *b;       // null for SVM instances (the residual is w = A^T (alpha y) / (lambda n))
   double *r64;           // FP64 residual of the refresh when it does not fit in LDS
   void *drg;             // rows [lds_rows, m) of the correction residual (T), or null
   int32_t m, n;
   int32_t tau;           // coordinates per step of this instance
   int32_t lds_rows;      // rows [0, lds_rows) of the correction residual live in LDS
   int32_t nnz;
-  int32_t nonneg;        // 1: NNLS (x >= 0, no lambda), 0: Lasso
+  int32_t loss;          // kLasso, kNnls (x >= 0, no lambda) or kSvm (SVM: the
+                         // coordinates are samples, x is alpha, CoordRec.L is
+                         // y_j / (beta ||x_j||^2): its sign is the label)
   double scale;          // NNLS: ||A^T b||_inf, the KKT residual's scale
   uint64_t seed;
 };
 
+__device__ __forceinline__ double sign_of(double v) { return (double)((v > 0.0) - (v < 0.0)); }
+// a block-uniform double moved to scalar registers (a division leaves its
+// result in vector registers, where it would hold two of them per lane)
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // The coordinate step of an instance: the Lasso's soft threshold
-// (MulticoreL1RegSquareLoss.h:41-51) or, for an NNLS instance, the projected
-// step onto x >= 0 (prox_delta<kNnls>); the scatter factor is the delta.
-__device__ __forceinline__ double batch_delta(bool nonneg, double g, double xj, double Lj,
+// (MulticoreL1RegSquareLoss.h:41-51), for an NNLS instance the projected step
+// onto x >= 0 (prox_delta<kNnls>), for an SVM instance the box-clipped dual
+// step (MulticoreSVMDualLoss.h:119-141) with the label taken from the sign of
+// the stored L.  Returns the change of x_j.  SVM instances get kernels of
+// their own (the SVM template flag), the Lasso and NNLS share theirs (loss
+// tested per instance), so that no kernel carries the registers of a step it
+// never runs.
+template <bool SVM>
+__device__ __forceinline__ double batch_delta(int loss, double g, double xj, double Lj,
                                              const StepScalars &sc) {
   double c;
-  return nonneg ? prox_delta<kNnls>(g, xj, Lj, 0.0, sc, &c) : prox_delta<kLasso>(g, xj, Lj, 0.0, sc, &c);
+  if constexpr (SVM) {
+    (void)loss;
+    return prox_delta<kSvm>(g, xj, fabs(Lj), sign_of(Lj), sc, &c);
+  } else {
+    return loss == kNnls ? prox_delta<kNnls>(g, xj, Lj, 0.0, sc, &c)
+                         : prox_delta<kLasso>(g, xj, Lj, 0.0, sc, &c);
+  }
+}
+// the factor of the column in the residual update for a change d of x_j:
+// d itself, or d y_j / (lambda n) for the SVM's w
+template <bool SVM>
+__device__ __forceinline__ double batch_coef(double d, double Lj, const StepScalars &sc) {
+  if constexpr (SVM) return d * sign_of(Lj) * sc.inv_lambda_n;
+  return d;
+}
+// step scalars of one lambda of an instance with n coordinates
+template <bool SVM>
+__device__ __forceinline__ StepScalars batch_scalars(double lambda, int32_t n) {
+  StepScalars sc;
+  sc.lambda = lambda;
+  sc.lambda_n = SVM ? uniform(lambda * (double)n) : 0.0;
+  sc.inv_lambda_n = SVM ? uniform(1.0 / (lambda * (double)n)) : 0.0;
+  return sc;
 }
 
 struct BatchResult {
@@ -213,13 +253,14 @@ __device__ __forceinline__ DrView<T, FULL> dr_view(const BatchInstance &I, T *ld
 }
 
 // FP64 refresh + evaluation of one instance by one block: fills out (in
-// shared memory) and zeroes dr.  r64 lives in LDS (reusing the space of dr,
+// shared memory) and zeroes dr.  For an SVM instance the residual is
+// w = sum_j alpha_j y_j x_j / (lambda n) (b = 0) and gref_j = <x_j, w>.  r64 lives in LDS (reusing the space of dr,
 // which is rebuilt at the end) when m doubles fit, in the global r64 buffer
 // otherwise.  The scatter of the nonzero x_j gives each thread its own
 // columns; the gradient pass streams the (column, row, value) entries in
 // wave-contiguous ranges and sums columns by segmented scans.  Statistics
 // meet in thread 0 only (block_stats).
-template <typename T, bool LDSR, bool RL>
+template <typename T, bool LDSR, bool RL, bool SVM>
 __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, double lambda, T *lds,
                                                 double *sh, BatchResult *out, bool dense) {
   // a copy: through a reference every store below could alias the
@@ -237,8 +278,11 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
     else
       gadd_block(rg + row, v);
   };
+  constexpr bool svm = SVM;
+  // SVM: the factor of alpha_j y_j in w
+  const double inv_ln = svm ? uniform(1.0 / (lambda * (double)I.n)) : 0.0;
   for (int k = tid; k < I.m; k += kBatchBlock) {
-    const double v = -gld(I.b + k);
+    const double v = svm ? 0.0 : -gld(I.b + k);
     if (rl)
       rs[k] = v;
     else
@@ -270,6 +314,7 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
         nz += 1.0;
         const int j = j0 + u * kBatchBlock;
         const int32_t p0 = gld(&I.cs[j].p0), len = gld(&I.cs[j].len);
+        const double cf = svm ? xs[u] * sign_of(gld(&I.cs[j].L)) * inv_ln : xs[u];
         // four entries loaded before their adds (with r64 in global memory
         // the adds are atomics, and a load issued after one waits for it)
         for (int32_t q0 = 0; q0 < len; q0 += 4) {
@@ -278,7 +323,7 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
           for (int q = 0; q < 4; ++q) e[q] = gld(I.rent + p0 + min(q0 + q, len - 1));
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            if (q0 + q < len) radd(e[q].row, e[q].val * xs[u]);
+            if (q0 + q < len) radd(e[q].row, e[q].val * cf);
         }
       }
     }
@@ -315,6 +360,9 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
         double xv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) xv[u] = gld_agent(&I.cs[cur[u].col].x);
+        if (svm)
+#pragma unroll
+          for (int u = 0; u < U; ++u) xv[u] *= sign_of(gld(&I.cs[cur[u].col].L)) * inv_ln;
 #pragma unroll
         for (int u = 0; u < U; ++u)
           if (base + u * 64 + ln < e_hi && xv[u] != 0.0) radd(cur[u].row, cur[u].val * xv[u]);
@@ -424,24 +472,38 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
   for (int k = tid; k < I.m; k += kBatchBlock) {
     const double rk = rget(k);
     r2 += rk * rk;
-    br += gld(I.b + k) * rk;
+    if (!svm) br += gld(I.b + k) * rk;
   }
-  if (I.nonneg) {  // NNLS: the KKT residual max_j |min(x_j, g_j)| takes max|g|'s slot
+  if (!svm && I.loss == kNnls) {  // NNLS: the KKT residual max_j |min(x_j, g_j)| takes max|g|'s slot
     ginf = 0;
     for (int j = tid; j < I.n; j += kBatchBlock)
       ginf = fmax(ginf, fabs(fmin(gld_agent(&I.cs[j].x), gld(&I.cs[j].gref))));
   }
+  if constexpr (svm)  // SVM: the hinge losses sum_j max(0, 1 - y_j <x_j, w>) take b.r's slot
+    for (int j = tid; j < I.n; j += kBatchBlock) {
+      const double h = 1.0 - sign_of(gld(&I.cs[j].L)) * gld(&I.cs[j].gref);
+      br += h > 0.0 ? h : 0.0;
+    }
   double st[kRefreshStats] = {ginf, x1, nz, r2, br};
   block_stats(st, sh);  // its barrier also ends every read of r
   dr_view<T, LDSR>(I, lds).zero(I.m);
   if (tid == 0) {
-    if (I.nonneg) {
+    if (!svm && I.loss == kNnls) {
       // objective 1/2 ||Ax - b||^2 (no dual is formed: 0); rel_gap is the
       // KKT residual over ||A^T b||_inf
       out->primal = 0.5 * st[3];
       out->dual = 0.0;
       out->nnz_x = st[2];
       out->rel_gap = I.scale > 0 ? st[0] / I.scale : st[0];
+    } else if (svm) {
+      // primal lambda/2 ||w||^2 + mean hinge, dual -lambda/2 ||w||^2 +
+      // mean alpha (MulticoreSVMDualLoss.h:63-93)
+      const double P = 0.5 * lambda * st[3] + st[4] / (double)I.n;
+      const double D = -0.5 * lambda * st[3] + st[1] / (double)I.n;
+      out->primal = P;
+      out->dual = D;
+      out->nnz_x = st[2];
+      out->rel_gap = P != 0 ? (P - D) / fabs(P) : 0.0;
     } else {
       const double P = 0.5 * st[3] + lambda * st[1];
       const double s = st[0] > lambda ? lambda / st[0] : 1.0;
@@ -457,21 +519,21 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
 
 // r64 in LDS when its m doubles fit (then dr fits as well); one body per
 // case, so that no access is issued for the case not taken
-template <typename T, bool LDSR>
+template <typename T, bool LDSR, bool SVM>
 __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double lambda, T *lds,
                                            double *sh, int lds_doubles, BatchResult *out,
                                            bool dense) {
   if (LDSR && Ip->m <= lds_doubles)
-    batch_refresh_impl<T, LDSR, LDSR>(Ip, lambda, lds, sh, out, dense);
+    batch_refresh_impl<T, LDSR, LDSR, SVM>(Ip, lambda, lds, sh, out, dense);
   else
-    batch_refresh_impl<T, LDSR, false>(Ip, lambda, lds, sh, out, dense);
+    batch_refresh_impl<T, LDSR, false, SVM>(Ip, lambda, lds, sh, out, dense);
 }
 
 // nsteps synchronous steps of one instance; S slots per G-lane group.  The
 // sampled coordinates of a wave's S * (64/G) slots are computed by its first
 // lanes in one pass and broadcast; each slot's record and column are loaded
 // after the first barrier of the previous step (its x writes are done then).
-template <typename T, int G, int S, int R, bool LDSR>
+template <typename T, int G, int S, int R, bool LDSR, bool SVM>
 __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambda, uint64_t step0,
                             uint64_t nsteps, T *lds) {
   constexpr int NG = kBatchBlock / G;  // groups in the block
@@ -484,10 +546,7 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
   const Perm perm = Perm::make((uint32_t)I.n);
   const Entry<T> *ent = static_cast<const Entry<T> *>(I.ent);
   const DrView<T, LDSR> dr = dr_view<T, LDSR>(I, lds);
-  StepScalars sc;
-  sc.lambda = lambda;
-  sc.lambda_n = 0;
-  sc.inv_lambda_n = 0;
+  const StepScalars sc = batch_scalars<SVM>(lambda, I.n);
   uint32_t j[S];
   CoordRec rec[S];
   ColCache<T, G, R> col[S];
@@ -522,8 +581,8 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
       T acc = col[k].dot(ent, dr, lane);
       acc = group_sum<G>(acc);
       const double g = rec[k].gref + (double)acc;
-      const double d = batch_delta(I.nonneg, g, rec[k].x, rec[k].L, sc);
-      c[k] = d;
+      const double d = batch_delta<SVM>(I.loss, g, rec[k].x, rec[k].L, sc);
+      c[k] = batch_coef<SVM>(d, rec[k].L, sc);
       if (lane == 0) *gp(&I.cs[j[k]].x) = rec[k].x + d;
     }
     __syncthreads();
@@ -547,7 +606,7 @@ __device__ __forceinline__ void batch_steps(const BatchInstance &I, double lambd
 // An instance's correction residual lives in LDS when the host gave it no
 // global buffer (drg == nullptr); each block branches on that per instance,
 // so one launch and one queue serve both placements.
-template <typename T, int G, int S, int R>
+template <typename T, int G, int S, int R, bool SVM>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
@@ -577,9 +636,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
     bool dense = false;
     auto refresh = [&](double lambda) {
       if (in_lds)
-        batch_refresh<T, true>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
+        batch_refresh<T, true, SVM>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
       else
-        batch_refresh<T, false>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
+        batch_refresh<T, false, SVM>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
       dense = 64.0 * s_res.nnz_x > (double)I.n;
     };
     const uint64_t steps_per_epoch = ((uint64_t)I.n + I.tau - 1) / I.tau;
@@ -594,9 +653,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
         const int ep = min(a.refresh_period, a.max_epochs - epochs);
         const uint64_t ns = steps_per_epoch * (uint64_t)ep;
         if (in_lds)
-          batch_steps<T, G, S, R, true>(I, lambda, step, ns, lds);
+          batch_steps<T, G, S, R, true, SVM>(I, lambda, step, ns, lds);
         else
-          batch_steps<T, G, S, R, false>(I, lambda, step, ns, lds);
+          batch_steps<T, G, S, R, false, SVM>(I, lambda, step, ns, lds);
         step += ns;
         epochs += ep;
         refresh(lambda);
@@ -632,7 +691,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
 // those loads, so that waiting for them does not wait for the prefetches
 // (memory counters drain in issue order).  A worker is then bound by about
 // one memory round trip per update, and a CU by its workers' count.
-template <typename T, int G, int R, bool LDSR, bool TAIL>
+template <typename T, int G, int R, bool LDSR, bool TAIL, bool SVM>
 __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, double lambda,
                                                     uint64_t k0, uint64_t count, T *lds) {
   const int lane = threadIdx.x & (G - 1);
@@ -640,10 +699,7 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
   if ((int32_t)w >= I.tau || count == 0) return;
   const Entry<T> *ent = static_cast<const Entry<T> *>(I.ent);
   const DrView<T, LDSR> dr = dr_view<T, LDSR>(I, lds);
-  StepScalars sc;
-  sc.lambda = lambda;
-  sc.lambda_n = 0;
-  sc.inv_lambda_n = 0;
+  const StepScalars sc = batch_scalars<SVM>(lambda, I.n);
   const bool serial = I.tau == 1;
   // this worker's last two updates (coordinate, delta)
   uint32_t j1 = 0xffffffffu, j2 = 0xffffffffu;
@@ -706,19 +762,20 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
     acc = group_sum<G>(acc);
     const double g = cur.r.gref + (double)acc;
     double d;
-    if (!I.nonneg) {
-      d = batch_delta(false, g, xj, cur.r.L, sc);
+    if (!SVM && I.loss == kLasso) {
+      d = batch_delta<false>(kLasso, g, xj, cur.r.L, sc);
       if (lane == 0 && d != 0.0) gadd_agent(&I.cs[cur.j].x, d);
     } else {
-      // x >= 0 under races: the new value is computed from the x_j it
-      // replaces, by compare-and-swap, and the scatter takes the delta that
-      // was swapped in (one worker: the first swap succeeds)
+      // x >= 0 (NNLS) or alpha in [0, 1] (SVM) under races: the new value is
+      // computed from the x_j it replaces, by compare-and-swap, and the
+      // scatter takes the delta that was swapped in (one worker: the first
+      // swap succeeds)
       d = 0.0;
       if (lane == 0) {
         unsigned long long *px = reinterpret_cast<unsigned long long *>(&I.cs[cur.j].x);
         double old = xj;
         for (;;) {
-          const double dd = batch_delta(true, g, old, cur.r.L, sc);
+          const double dd = batch_delta<SVM>(kNnls, g, old, cur.r.L, sc);  // NNLS or SVM
           const double nw = old + dd;
           if (nw == old) {
             d = dd;
@@ -735,7 +792,7 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
       }
       d = __shfl(d, 0, G);
     }
-    const double c = d;
+    const double c = batch_coef<SVM>(d, cur.r.L, sc);
     j2 = j1;
     d2 = d1;
     j1 = cur.j;
@@ -768,7 +825,7 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
   if (k < count) update(k, s0, s1, ahead);
 }
 
-template <typename T, int G, int R, bool TAIL>
+template <typename T, int G, int R, bool TAIL, bool SVM>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
@@ -796,9 +853,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
     bool dense = false;
     auto refresh = [&](double lambda) {
       if (in_lds)
-        batch_refresh<T, true>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
+        batch_refresh<T, true, SVM>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
       else
-        batch_refresh<T, false>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
+        batch_refresh<T, false, SVM>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
       dense = 64.0 * s_res.nnz_x > (double)I.n;
     };
     uint64_t k = 0;  // updates per worker so far
@@ -812,9 +869,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
         const int ep = min(a.refresh_period, a.max_epochs - epochs);
         const uint64_t per = ((uint64_t)I.n * (uint64_t)ep + I.tau - 1) / I.tau;
         if (in_lds)
-          batch_async_updates<T, G, R, true, TAIL>(I, lambda, k, per, lds);
+          batch_async_updates<T, G, R, true, TAIL, SVM>(I, lambda, k, per, lds);
         else
-          batch_async_updates<T, G, R, false, TAIL>(I, lambda, k, per, lds);
+          batch_async_updates<T, G, R, false, TAIL, SVM>(I, lambda, k, per, lds);
         __syncthreads();  // every worker's updates are in before the refresh
         k += per;
         epochs += ep;
@@ -838,5 +895,48 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
     }
   }
 }
+
+// ------------------------------------------------------ kernel selection
+// coordinates per lane group and step of the synchronous kernel; with 4
+// register chunks per column this keeps every instantiation within 128
+// VGPRs (1024-thread blocks) without scratch spills
+constexpr int kSlotsPerGroup = 2;
+
+// column entries cached in registers per lane: 4 chunks, 2 for 64-lane groups
+constexpr int regs_for(int lanes) { return lanes == 64 ? 2 : 4; }
+
+// entries of a column an asynchronous worker caches in registers per lane
+// (FP32: 8 chunks for 8 and 16 lanes, 4 for 32, 2 for 64; FP64: at most 4),
+// the most that stays spill-free in 128 VGPRs; longer columns take the
+// variant with a tail loop
+template <typename T>
+constexpr int async_regs_for(int lanes) {
+  return lanes == 64 ? 2 : (lanes == 32 || sizeof(T) == 8 ? 4 : 8);
+}
+
+template <typename T, int G, bool SVM>
+const void *batch_fn(bool async, bool tail) {
+  constexpr int RA = async_regs_for<T>(G);
+  if (async)
+    return tail ? reinterpret_cast<const void *>(&k_batch_async<T, G, RA, true, SVM>)
+                : reinterpret_cast<const void *>(&k_batch_async<T, G, RA, false, SVM>);
+  return reinterpret_cast<const void *>(&k_batch_lasso<T, G, kSlotsPerGroup, regs_for(G), SVM>);
+}
+
+// the batch kernel for a precision, lanes per coordinate (8, 16, 32 or 64),
+// step mode and longest column; SVM = true is instantiated in a translation
+// unit of its own and reached through batch_svm_kernel
+template <bool SVM>
+const void *batch_kernel_for(bool fp32, int lanes, bool async, int64_t maxlen) {
+  const int ra = fp32 ? async_regs_for<float>(lanes) : async_regs_for<double>(lanes);
+  const bool tail = maxlen > (int64_t)ra * lanes;
+  switch (lanes) {
+    case 8: return fp32 ? batch_fn<float, 8, SVM>(async, tail) : batch_fn<double, 8, SVM>(async, tail);
+    case 16: return fp32 ? batch_fn<float, 16, SVM>(async, tail) : batch_fn<double, 16, SVM>(async, tail);
+    case 32: return fp32 ? batch_fn<float, 32, SVM>(async, tail) : batch_fn<double, 32, SVM>(async, tail);
+    default: return fp32 ? batch_fn<float, 64, SVM>(async, tail) : batch_fn<double, 64, SVM>(async, tail);
+  }
+}
+const void *batch_svm_kernel(bool fp32, int lanes, bool async, int64_t maxlen);
 
 }  // namespace acdc

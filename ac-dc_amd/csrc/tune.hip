@@ -20,8 +20,10 @@
 
 namespace {
 
-// ||A^T b||_inf of a column-major problem
+// ||A^T b||_inf of a column-major problem (for sample rows, the SVM's
+// labels: 10 / n, so that frac = 0.1 gives lambda = 1 / n)
 double lambda_max(const acdc_problem *p) {
+  if (p->kind == 1) return 10.0 / (double)std::max<int64_t>(p->ncoord, 1);
   double m = 0;
   for (int64_t j = 0; j < p->ncoord; ++j) {
     double g = 0;
@@ -45,9 +47,11 @@ extern "C" acdc_status acdc_batch_tune(const acdc_problem *const *problems, int3
     return tfail(ACDC_ERR_INVALID, "acdc_batch_tune: null argument or empty batch");
   // the instance that bounds the batch: most entries
   int32_t big = 0;
+  const int kind = base->loss == ACDC_LOSS_SVM_DUAL_HINGE ? 1 : 0;
   for (int32_t i = 0; i < count; ++i) {
-    if (!problems[i] || problems[i]->kind != 0)
-      return tfail(ACDC_ERR_INVALID, "acdc_batch_tune: every problem must be column-major (Lasso, NNLS)");
+    if (!problems[i] || problems[i]->kind != kind)
+      return tfail(ACDC_ERR_INVALID, kind ? "acdc_batch_tune: SVM problems must be sample rows (CSR)"
+                                          : "acdc_batch_tune: Lasso and NNLS problems must be column-major");
     if (problems[i]->nnz > problems[big]->nnz) big = i;
   }
   const acdc_problem *p = problems[big];

@@ -218,17 +218,17 @@ acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps,
 /* Waits for all work queued on the solver's stream. */
 acdc_status acdc_synchronize(acdc_solver *s);
 
-/* ---- batched Lasso: many instances x warm-started lambda paths ----------
- * One persistent launch (two when some instances have more rows than the
- * LDS residual holds) solves every instance along its lambda path: lambda l+1
- * starts from the x of lambda l (lasso_console_solver.cpp:149-172), and each
- * (instance, lambda) stops at FP64 relative duality gap <= gap_tol (checked
- * with an FP64 refresh every refresh_period epochs, mSDCA.cpp:123 stops on
- * the gap the same way) or after max_epochs.  Instances are handed to
- * workgroups by a device work queue, largest first.  Instance i samples
- * with key seed + i and tau = 4 * (1024 / lanes_per_coord) coordinates per
- * step (capped at its n); its ESO beta comes from its omega.  No counterpart
- * in the reference beyond the lambda-path driver it generalises. */
+/* ---- batched Lasso / NNLS / SVM: many instances x warm-started lambda paths
+ * One persistent launch solves every instance along its lambda path: lambda
+ * l+1 starts from the x of lambda l (lasso_console_solver.cpp:149-172), and
+ * each (instance, lambda) stops at FP64 relative duality gap <= gap_tol
+ * (checked with an FP64 refresh every refresh_period epochs, mSDCA.cpp:123
+ * stops on the gap the same way) or after max_epochs.  Instances are handed
+ * to workgroups by a device work queue, largest first.  Instance i samples
+ * with key seed + i and by default tau = 2 * (1024 / lanes_per_coord)
+ * coordinates per synchronous step, or 1024 / lanes_per_coord asynchronous
+ * workers (capped at its n); its ESO beta comes from its omega.  No
+ * counterpart in the reference beyond the lambda-path driver it generalises. */
 typedef struct acdc_batch acdc_batch;
 
 typedef struct acdc_batch_settings {
@@ -249,11 +249,14 @@ typedef struct acdc_batch_settings {
   int32_t queue_chunk;     /* instances a block takes from the work queue per
                               access (0 = 1); consecutive in the queue's
                               largest-first order */
-  int32_t loss;            /* of every instance: ACDC_LOSS_L1_SQUARE (0) or
+  int32_t loss;            /* of every instance: ACDC_LOSS_L1_SQUARE (0),
                               ACDC_LOSS_NNLS (x >= 0, the lambdas are ignored;
                               primal = 1/2 ||Ax-b||^2, dual = 0, rel_gap = the
                               KKT residual max_j |min(x_j, g_j)| over
-                              ||A^T b||_inf) */
+                              ||A^T b||_inf) or ACDC_LOSS_SVM_DUAL_HINGE
+                              (problems from acdc_problem_create_csr, lambdas
+                              > 0; x is alpha; primal and dual as
+                              acdc_report's; nnz_x counts alpha_j != 0) */
 } acdc_batch_settings;
 
 typedef struct acdc_batch_result {
@@ -279,7 +282,8 @@ acdc_status acdc_batch_destroy(acdc_batch *b);
 /* ---- autotuner (no reference counterpart: the reference fixes tau by the
  * thread count and has no lanes-per-coordinate or queue knobs) ------------
  * acdc_batch_tune times short runs of the batch's largest instance (4
- * epochs at lambda = 0.1 lambda_max) for lanes per coordinate in {8,16,32}
+ * epochs at lambda = 0.1 lambda_max, or 1 / n for SVM batches: the loss
+ * comes from base->loss) for lanes per coordinate in {8,16,32}
  * and tau in {cap/4, cap/2, cap} (cap: the block's slots or workers for
  * those lanes) and returns *base with the lanes and tau of the best score =
  * log(gap at x = 0 / gap after the trial) per second of device time, and

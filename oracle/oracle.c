@@ -19,7 +19,9 @@
  *   orc_lasso_eval      gap >= 0, gap = 0 at x = 0 when lambda >= lambda_max,
  *                       closed form gap(0) = ||b||^2 (1-s)^2/2.
  *   orc_svm_*           scipy L-BFGS-B on the box-constrained dual (golden),
- *                       brute force for n <= 8, P >= D, w identity.
+ *                       brute force for n <= 8, P >= D, w identity;
+ *                       orc_svm_list equals orc_svm_run(tau=1) on the
+ *                       samples orc_sample_steps gives it (bit for bit).
  *   orc_async_coords    multiply-shift map checked against exact integer
  *                       arithmetic; chi^2 uniformity; per-worker streams
  *                       distinct.
@@ -412,6 +414,23 @@ void orc_svm_run(int64_t ns, int64_t nf, const int32_t *row_ptr,
   }
   free(S);
   free(d);
+}
+
+/* serial SDCA steps on the given samples, in order (the one-worker case of
+ * the asynchronous executor, MulticoreEngineExecutor.h:83-89, with
+ * MulticoreSVMDualLoss.h:119-141 as the step) */
+void orc_svm_list(int64_t ns, int64_t nf, const int32_t *row_ptr, const int32_t *col_idx,
+                  const double *vals, const double *y, const double *L, double lambda,
+                  const int32_t *coords, uint64_t count, double *alpha, double *w) {
+  (void)nf;
+  const double delta = 1.0 / (lambda * (double)ns);
+  for (uint64_t k = 0; k < count; ++k) {
+    const int32_t j = coords[k];
+    const double d = svm_delta(row_ptr, col_idx, vals, y, L, lambda, ns, alpha, w, j);
+    alpha[j] += d;
+    for (int32_t p = row_ptr[j]; p < row_ptr[j + 1]; ++p)
+      w[col_idx[p]] += d * vals[p] * y[j] * delta;
+  }
 }
 
 void orc_svm_eval(int64_t ns, int64_t nf, const int32_t *row_ptr,

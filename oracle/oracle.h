@@ -102,6 +102,10 @@ void orc_svm_run(int64_t ns, int64_t nf, const int32_t *row_ptr,
 /* out[0..7] = {gap, primal, dual, train 0-1 accuracy, rel_gap, ||w||^2,
  *              sum alpha, sum hinge}: MulticoreSVMDualLoss.h:63-117 with w
  * recomputed from alpha in FP64 */
+/* serial SDCA steps on coords[0..count) in order */
+void orc_svm_list(int64_t ns, int64_t nf, const int32_t *row_ptr, const int32_t *col_idx,
+                  const double *vals, const double *y, const double *L, double lambda,
+                  const int32_t *coords, uint64_t count, double *alpha, double *w);
 void orc_svm_eval(int64_t ns, int64_t nf, const int32_t *row_ptr,
                   const int32_t *col_idx, const double *vals, const double *y,
                   double lambda, const double *alpha, double out[8]);

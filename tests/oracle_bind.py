@@ -48,6 +48,8 @@ def lib() -> ctypes.CDLL:
             "orc_svm_run": ([_i64, _i64, _I32P, _I32P, _F64P, _F64P, _F64P, _f64, _u64, _i64, _u64,
                              _u64, _F64P, _F64P], None),
             "orc_svm_eval": ([_i64, _i64, _I32P, _I32P, _F64P, _F64P, _f64, _F64P, _F64P], None),
+            "orc_svm_list": ([_i64, _i64, _I32P, _I32P, _F64P, _F64P, _F64P, _f64, _I32P, _u64,
+                              _F64P, _F64P], None),
             "orc_cd_async_omp": ([ctypes.c_int, _i64, _i64, _I32P, _I32P, _F64P, _F64P, _f64, _u64,
                                   _u64, _u64, ctypes.c_int, _F64P, _F64P], ctypes.c_int),
             "orc_async_coords": ([_u64, ctypes.c_uint32, _u64, _u64, ctypes.c_uint32, _I32P], None),
@@ -217,6 +219,12 @@ class SVM:
         lib().orc_svm_run(self.ns, self.nf, *self._args(), _p(self.L, ctypes.c_double), self.lam,
                           seed, tau, step0, nsteps, _p(self.alpha, ctypes.c_double),
                           _p(self.w, ctypes.c_double))
+
+    def run_list(self, coords):
+        c = i32(coords)
+        lib().orc_svm_list(self.ns, self.nf, *self._args(), _p(self.L, ctypes.c_double), self.lam,
+                           _p(c, ctypes.c_int32), c.size, _p(self.alpha, ctypes.c_double),
+                           _p(self.w, ctypes.c_double))
 
     def eval(self, alpha=None) -> dict:
         a = self.alpha if alpha is None else f64(alpha)

@@ -66,6 +66,27 @@ def test_solver_argument_errors(acdc):
                     launch_mode=acdc.LAUNCH_PERSISTENT)
 
 
+def test_batch_argument_errors(acdc):
+    """acdc_batch_create checks its arguments before it looks for a device."""
+    csc = acdc.Problem.from_csc(3, 2, [0, 2, 3], [0, 2, 2], [1.0, 2.0, 3.0], np.ones(3))
+    csr = acdc.Problem.from_csr(2, 3, [0, 2, 3], [0, 2, 2], [1.0, 2.0, 3.0], [1.0, -1.0])
+    lam = np.full((1, 2), 0.1)
+    with pytest.raises(acdc.AcdcError, match="loss must be"):
+        acdc.Batch([csc], lam, loss=1)
+    with pytest.raises(acdc.AcdcError, match="SVM problems must be sample rows"):
+        acdc.Batch([csc], lam, loss=acdc.LOSS_SVM_DUAL_HINGE)
+    with pytest.raises(acdc.AcdcError, match="must be column-major"):
+        acdc.Batch([csr], lam, loss=acdc.LOSS_L1_SQUARE)
+    with pytest.raises(acdc.AcdcError, match="SVM lambdas must be > 0"):
+        acdc.Batch([csr], np.array([[0.1, 0.0]]), loss=acdc.LOSS_SVM_DUAL_HINGE)
+    with pytest.raises(acdc.AcdcError, match="lambdas must be >= 0"):
+        acdc.Batch([csc], np.array([[0.1, -1.0]]))
+    with pytest.raises(acdc.AcdcError, match="lanes_per_coord"):
+        acdc.Batch([csc], lam, lanes_per_coord=12)
+    with pytest.raises(acdc.AcdcError, match="SVM problems must be sample rows"):
+        acdc.batch_tune([csc], loss=acdc.LOSS_SVM_DUAL_HINGE)
+
+
 def test_config1_generator(acdc):
     d = acdc.config_lasso_c1()
     assert (d.m, d.n, d.nnz) == (2000, 10000, 200000)

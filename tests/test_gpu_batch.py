@@ -1,4 +1,4 @@
-"""Batched multi-instance Lasso (acdc_batch_*) against the FP64 oracle.
+"""Batched multi-instance Lasso, NNLS and SVM (acdc_batch_*) against the FP64 oracle.
 
 Instance i of a batch samples with key seed + i and its own tau and ESO beta,
 so its trajectory is the oracle's synchronous run with those parameters; the
@@ -252,3 +252,119 @@ def test_batch_nnls_async_workers_converge(acdc, orc, ttd_batch):
         assert e["kkt"] / nnls_scale(d) <= 1e-4
         assert res[i, 0]["primal"] == pytest.approx(e["primal"], rel=1e-9)
     b.close()
+
+
+@pytest.fixture(scope="module")
+def svm_batch(acdc):
+    # hinge-SVM dual instances (coordinates = samples, residual = w over the
+    # features); the last one's w does not fit the LDS residual
+    shapes = [(400, 3000, 30.0), (1500, 20000, 60.0), (900, 50000, 80.0)]
+    return [acdc.gen_svm(ns, nf, row_median=med, row_sigma=0.9, wstar_nnz=nf // 10,
+                         flip_frac=0.05, seed=200 + i)
+            for i, (ns, nf, med) in enumerate(shapes)]
+
+
+def svm_lambdas(ds):
+    # a warm-started path 10/n -> 1/n per instance
+    return np.stack([np.array([10.0, 3.0, 1.0]) / d.n for d in ds])
+
+
+def svm_oracle(orc, d, lam, beta=1.0):
+    return orc.SVM(d.n, d.m, d.ptr, d.idx, d.vals, d.rhs, lam, beta=beta)
+
+
+@pytest.mark.parametrize("lanes", [16, 32])
+def test_batch_svm_sync_matches_oracle(acdc, orc, svm_batch, lanes):
+    """SVM instances, synchronous FP64 steps at fixed epochs along the path:
+    the oracle's box-clipped SDCA steps with the same sampling, tau and
+    beta (w recomputed from alpha at each new lambda), and the refresh's
+    primal, dual and relative gap."""
+    ds = svm_batch
+    lams = svm_lambdas(ds)
+    probs = [d.problem() for d in ds]
+    epochs = 2
+    b = acdc.Batch(probs, lams, seed=80, precision=acdc.PREC_FP64, lanes_per_coord=lanes,
+                   refresh_period=1, max_epochs=epochs, gap_tol=-1.0, use_graph=False,
+                   loss=acdc.LOSS_SVM_DUAL_HINGE)
+    b.run()
+    res = b.results()
+    assert (res["epochs"] == epochs).all()
+    assert not b.info(2)[2]  # the widest instance's w lives in global memory
+    for i, d in enumerate(ds):
+        tau, beta, _ = b.info(i)
+        o = svm_oracle(orc, d, lams[i, 0], beta)
+        steps = epochs * (-(-d.n // tau))
+        step0 = 0
+        for lam in lams[i]:
+            o.lam = float(lam)
+            o.recompute_w()
+            o.run(seed=80 + i, tau=tau, step0=step0, nsteps=steps)
+            step0 += steps
+        a = b.x(i)
+        assert a.min() >= 0.0 and a.max() <= 1.0
+        assert np.max(np.abs(a - o.alpha)) < 1e-10, i
+        e = o.eval()
+        assert res[i, -1]["primal"] == pytest.approx(e["primal"], rel=1e-10)
+        assert res[i, -1]["dual"] == pytest.approx(e["dual"], rel=1e-10)
+        assert res[i, -1]["rel_gap"] == pytest.approx(e["rel_gap"], rel=1e-7, abs=1e-13)
+        assert abs(res[i, -1]["nnz_x"] - np.count_nonzero(o.alpha)) <= 2
+    b.close()
+
+
+def test_batch_svm_async_one_worker_is_serial(acdc, orc, svm_batch):
+    """ASYNC SVM instances with tau = 1: the worker's stream replayed by the
+    serial oracle (orc_svm_list) along the warm-started path, at FP64."""
+    ds = svm_batch
+    lams = svm_lambdas(ds)
+    probs = [d.problem() for d in ds]
+    b = acdc.Batch(probs, lams, seed=90, precision=acdc.PREC_FP64, refresh_period=1,
+                   max_epochs=2, gap_tol=-1.0, use_graph=False, step_mode=acdc.STEP_ASYNC,
+                   tau=1, loss=acdc.LOSS_SVM_DUAL_HINGE)
+    b.run()
+    res = b.results()
+    for i, d in enumerate(ds):
+        assert b.info(i)[0] == 1
+        o = svm_oracle(orc, d, lams[i, 0])
+        k = 0
+        for lam in lams[i]:
+            o.lam = float(lam)
+            o.recompute_w()
+            o.run_list(orc.async_coords(90 + i, 0, k, 2 * d.n, d.n))
+            k += 2 * d.n
+        a = b.x(i)
+        assert np.max(np.abs(a - o.alpha)) < 1e-10, i
+        assert res[i, -1]["primal"] == pytest.approx(o.eval()["primal"], rel=1e-10)
+    b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batch_svm_async_workers_converge(acdc, orc, svm_batch, prec):
+    """Many workers with compare-and-swap steps on the box [0, 1]: every
+    (instance, lambda) reaches relative gap 1e-5, which the FP64 oracle's
+    evaluation of the device's alpha confirms."""
+    ds = svm_batch
+    lams = svm_lambdas(ds)
+    probs = [d.problem() for d in ds]
+    precision = acdc.PREC_FP32_REFRESH if prec == "fp32" else acdc.PREC_FP64
+    b = acdc.Batch(probs, lams, seed=91, precision=precision, refresh_period=5,
+                   max_epochs=2000, gap_tol=1e-5, use_graph=True, step_mode=acdc.STEP_ASYNC,
+                   loss=acdc.LOSS_SVM_DUAL_HINGE)
+    b.run()
+    res = b.results()
+    assert res["converged"].all(), res["rel_gap"]
+    for i, d in enumerate(ds):
+        a = b.x(i)
+        assert a.min() >= 0.0 and a.max() <= 1.0
+        e = svm_oracle(orc, d, lams[i, -1]).eval(a)
+        assert e["rel_gap"] <= 1.0001e-5
+        assert res[i, -1]["primal"] == pytest.approx(e["primal"], rel=1e-9)
+        assert res[i, -1]["dual"] == pytest.approx(e["dual"], rel=1e-9)
+    b.close()
+
+
+def test_batch_svm_autotuner(acdc, svm_batch):
+    probs = [d.problem() for d in svm_batch]
+    t = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC, loss=acdc.LOSS_SVM_DUAL_HINGE)
+    assert t["lanes_per_coord"] in (8, 16, 32)
+    assert 1 <= t["tau"] <= 1024 // t["lanes_per_coord"]
+    assert t["score"] > 0

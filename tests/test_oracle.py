@@ -316,6 +316,23 @@ def test_svm_w_identity(orc):
     assert np.allclose(s.w, (a * y) @ X / (0.05 * ns), rtol=1e-14, atol=1e-16)
 
 
+def test_svm_list_equals_serial_run(orc):
+    rng = np.random.default_rng(14)
+    ns, nf = 60, 40
+    X = (rng.random((ns, nf)) < 0.25) * rng.standard_normal((ns, nf))
+    X[3] = 0.0
+    rp = np.concatenate([[0], np.cumsum((X != 0).sum(1))]).astype(np.int32)
+    ci = np.nonzero(X)[1].astype(np.int32)
+    v = X[X != 0]
+    y = rng.choice([-1.0, 1.0], ns)
+    a = orc.SVM(ns, nf, rp, ci, v, y, 0.02)
+    s = orc.SVM(ns, nf, rp, ci, v, y, 0.02)
+    a.run_list(orc.sample_steps(6, 0, 2000, ns, 1))
+    s.run(seed=6, tau=1, step0=0, nsteps=2000)
+    assert (a.alpha == s.alpha).all() and (a.w == s.w).all()
+    assert a.alpha.min() >= 0 and a.alpha.max() <= 1 and a.alpha[3] == 0
+
+
 def test_async_coords_multiply_shift_exact(orc):
     """The 64x32-bit multiply-shift map against exact integer arithmetic on
     the Philox words, and distinct workers get distinct streams."""
