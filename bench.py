@@ -227,6 +227,159 @@ def batched(acdc, args):
     return out
 
 
+def nnls_scale(d):
+    """||A^T b||_inf of a column-major dataset: the NNLS KKT residual's scale."""
+    import numpy as np
+    g = np.zeros(d.n)
+    np.add.at(g, np.repeat(np.arange(d.n), np.diff(d.ptr)), d.vals * d.rhs[d.idx])
+    return float(np.max(np.abs(g))) or 1.0
+
+
+def batched_plain_sample(acdc, args, ds, probs, lams, loss, kw, sample, fixed_epochs=0):
+    """Plain FP64 path (synchronous per-step launches, one instance at a
+    time, gap checked every 5 epochs as the batch refreshes) against the
+    batched kernel alone on the same instances and lambdas.  fixed_epochs > 0:
+    both run that many epochs instead of stopping at the gap target."""
+    rows = []
+    for i in sample:
+        d, p = ds[i], probs[i]
+        P = lams.shape[1]
+        bi = acdc.Batch([p], lams[i:i + 1], use_graph=False, **kw)
+        bi.run()
+        tb = bi.run() * 1e-3
+        ri = bi.results()
+        tau = int(bi.info(0)[0])
+        bi.close()
+        s = acdc.Solver(p, loss, lam=float(lams[i, 0]), tau=tau, seed=5,
+                        precision=acdc.PREC_FP64, launch_mode=acdc.LAUNCH_PER_STEP)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for l in range(P):
+            s.set_lambda(float(lams[i, l]))
+            ep, rep = 0, s.report()
+            cap = fixed_epochs or kw["max_epochs"]
+            while (fixed_epochs or rep["rel_gap"] > kw["gap_tol"]) and ep < cap:
+                for _ in range(5):
+                    s.run_updates(d.n)
+                ep += 5
+                rep = s.report()
+        tp = time.perf_counter() - t0
+        up = rep["updates"]
+        s.close()
+        rows.append({"instance": i, "m": d.m, "n": d.n, "nnz": d.nnz, "lambdas": P,
+                     "batched_s": tb, "batched_updates": int(ri["updates"][0, -1]),
+                     "batched_rel_gap": float(ri["rel_gap"][0, -1]),
+                     "fp64_plain_s": tp, "fp64_plain_updates": int(up),
+                     "fp64_plain_rel_gap": float(rep["rel_gap"] if loss != acdc.LOSS_NNLS
+                                                 else rep["kkt"] / nnls_scale(d)),
+                     "speedup": tp / tb})
+        log(f"[bench] batched sample {i} ({d.m}x{d.n}): batched {tb:.3f}s vs plain fp64 "
+            f"{tp:.3f}s = {tp / tb:.1f}x")
+    return rows
+
+
+def plain_estimate(out, updates, wall):
+    """The whole batch at the plain FP64 path's measured cost per update (the
+    sampled instances run one after another on the whole device)."""
+    import numpy as np
+    rows = out["fp64_plain_sampled"]
+    per = [r["fp64_plain_s"] / max(1, r["fp64_plain_updates"]) for r in rows]
+    out["fp64_plain_estimate_s"] = float(np.mean(per) * updates)
+    out["speedup_vs_fp64_plain_estimate"] = out["fp64_plain_estimate_s"] / wall
+
+
+def batched_svm(acdc, args):
+    """Batched hinge-SVM duals: 256 instances of uneven sizes (samples
+    log-uniform in [1k, 8k], features in [10k, 200k], row nnz lognormal with
+    median log-uniform in [20, 200]), each along an 8-value warm-started path
+    lambda = 10/n -> 1/n, stopping at FP64 relative duality gap <= the gap
+    target or 100 epochs per lambda; one graph-captured launch (FP32 + FP64
+    refresh of w every 5 epochs, asynchronous workers with compare-and-swap
+    steps on [0, 1]); beside it the plain FP64 path on three sampled
+    instances over the same paths."""
+    import numpy as np
+    t = time.perf_counter()
+    rng = np.random.default_rng(6)
+    count = 256
+    ns = np.exp(rng.uniform(np.log(1000), np.log(8000), count)).astype(np.int64)
+    nf = np.exp(rng.uniform(np.log(10000), np.log(200000), count)).astype(np.int64)
+    med = np.exp(rng.uniform(np.log(20), np.log(200), count))
+    ds = [acdc.gen_svm(int(a), int(f), row_median=float(r), row_sigma=0.9,
+                       wstar_nnz=max(1, int(f) // 100), flip_frac=0.05, seed=6000 + i)
+          for i, (a, f, r) in enumerate(zip(ns, nf, med))]
+    nnz = np.array([d.nnz for d in ds])
+    log(f"[bench] batched svm: generated {count} instances, nnz {nnz.sum()} in "
+        f"{time.perf_counter() - t:.1f}s")
+    probs = [d.problem() for d in ds]
+    lams = np.stack([np.geomspace(10.0, 1.0, 8) / d.n for d in ds])
+    tuned = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC, loss=acdc.LOSS_SVM_DUAL_HINGE)
+    kw = dict(seed=5, precision=acdc.PREC_FP32_REFRESH, refresh_period=5, max_epochs=100,
+              gap_tol=args.gap_target, step_mode=acdc.STEP_ASYNC,
+              lanes_per_coord=tuned["lanes_per_coord"], tau=tuned["tau"],
+              queue_chunk=tuned["queue_chunk"], loss=acdc.LOSS_SVM_DUAL_HINGE)
+    b = acdc.Batch(probs, lams, use_graph=True, **kw)
+    t0 = time.perf_counter()
+    ms = b.run()
+    wall = time.perf_counter() - t0
+    r = b.results()
+    upd = float(r["updates"][:, -1].astype(np.float64).sum())
+    out = {"workload": f"{count} svm dual instances x 8 lambdas (10/n -> 1/n)",
+           "nnz": int(nnz.sum()), "max_epochs_per_lambda": 100, "gap_tol": args.gap_target,
+           "device_ms": ms, "wall_s": wall, "updates": upd, "value": upd / wall,
+           "unit": "coord updates/s", "pairs_converged": float(r["converged"].mean()),
+           "worst_rel_gap": float(r["rel_gap"].max()), "autotuned": tuned}
+    log(f"[bench] batched svm: {ms:.0f} ms device, {upd:.3e} updates = {upd / wall:.3e} upd/s, "
+        f"{out['pairs_converged']:.3f} of pairs converged")
+    b.close()
+    order = np.argsort(nnz)
+    sample = [int(order[int(q * (count - 1))]) for q in (0.25, 0.5, 0.75)]
+    out["fp64_plain_sampled"] = batched_plain_sample(acdc, args, ds, probs, lams,
+                                                     acdc.LOSS_SVM_DUAL_HINGE, kw, sample)
+    plain_estimate(out, upd, wall)
+    return out
+
+
+def batched_ttd(acdc, args):
+    """Batched truss-topology NNLS: 32 ground structures of 4^3 to 15^3 cells
+    (1,000 to 55,000 bars), 200 epochs each in one graph-captured launch
+    (FP32 + FP64 refresh every 5 epochs, asynchronous workers with
+    compare-and-swap steps on x >= 0); beside it the plain FP64 path on three
+    sampled instances for the same 200 epochs.  The relative KKT residual
+    after the epochs is reported for both."""
+    import numpy as np
+    t = time.perf_counter()
+    count, epochs = 32, 200
+    ds = [acdc.gen_ttd(4 + (i * 7) % 12) for i in range(count)]
+    nnz = np.array([d.nnz for d in ds])
+    log(f"[bench] batched ttd: generated {count} instances, nnz {nnz.sum()} in "
+        f"{time.perf_counter() - t:.1f}s")
+    probs = [d.problem() for d in ds]
+    lams = np.zeros((count, 1))
+    tuned = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC, loss=acdc.LOSS_NNLS)
+    kw = dict(seed=5, precision=acdc.PREC_FP32_REFRESH, refresh_period=5, max_epochs=epochs,
+              gap_tol=-1.0, step_mode=acdc.STEP_ASYNC, lanes_per_coord=tuned["lanes_per_coord"],
+              tau=tuned["tau"], queue_chunk=tuned["queue_chunk"], loss=acdc.LOSS_NNLS)
+    b = acdc.Batch(probs, lams, use_graph=True, **kw)
+    t0 = time.perf_counter()
+    ms = b.run()
+    wall = time.perf_counter() - t0
+    r = b.results()
+    upd = float(r["updates"][:, -1].astype(np.float64).sum())
+    out = {"workload": f"{count} ttd nnls instances, {epochs} epochs each", "nnz": int(nnz.sum()),
+           "device_ms": ms, "wall_s": wall, "updates": upd, "value": upd / wall,
+           "unit": "coord updates/s", "median_rel_kkt": float(np.median(r["rel_gap"][:, 0])),
+           "worst_rel_kkt": float(r["rel_gap"].max()), "autotuned": tuned}
+    log(f"[bench] batched ttd: {ms:.0f} ms device, {upd:.3e} updates = {upd / wall:.3e} upd/s, "
+        f"median relative KKT {out['median_rel_kkt']:.2e}")
+    b.close()
+    order = np.argsort(nnz)
+    sample = [int(order[int(q * (count - 1))]) for q in (0.25, 0.5, 0.75)]
+    out["fp64_plain_sampled"] = batched_plain_sample(acdc, args, ds, probs, lams, acdc.LOSS_NNLS,
+                                                     kw, sample, fixed_epochs=epochs)
+    plain_estimate(out, upd, wall)
+    return out
+
+
 def svm_c3(acdc, args):
     """Hinge-SVM dual minibatch SDCA on 20,000 samples x 1,350,000 features
     (lognormal row nnz, median 300): synchronous minibatches b in {512, 2048,
@@ -445,6 +598,8 @@ def main():
 
     if not args.no_batch:
         result["batched"] = batched(acdc, args)
+        result["batched_svm"] = batched_svm(acdc, args)
+        result["batched_ttd"] = batched_ttd(acdc, args)
     if not args.no_svm_ttd:
         result["svm"] = svm_c3(acdc, args)
         result["ttd"] = ttd_c4(acdc, args)
