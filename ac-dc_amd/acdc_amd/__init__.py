@@ -38,6 +38,7 @@ STEP_ASYNC = 1
 LAUNCH_AUTO = 0
 LAUNCH_PER_STEP = 1
 LAUNCH_PERSISTENT = 2
+LAUNCH_PER_STEP_STREAMED = 3
 
 NNZ_FIXED = 0
 NNZ_ZIPF_TRUNC = 1

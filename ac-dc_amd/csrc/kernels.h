@@ -94,6 +94,11 @@ struct StepArgs {
   int64_t tau;
   int32_t *coord;
   double *coef;
+  // streamed scatter (k_step_scatter_t): two halves of ncoord coefficients,
+  // step s writes half s & 1 at its coordinates and clears those of step
+  // s - 1 in the other half; null when the scatter is atomic
+  T *cvec;
+  int64_t ncoord;
 };
 
 // One coordinate's proximal step from the group's reduced partial sum; every
@@ -151,6 +156,14 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(StepArgs<T> a, uint64_t 
   double c;
   const double d = coord_delta<T, LOSS>(a, j, (double)acc, xj, &c);
   if (lane == 0) {
+    if (a.cvec) {
+      // this slot's coordinate of the previous step leaves the other half
+      // (read before this step's coordinate replaces it); no other slot of
+      // this step writes there, since the other half belongs to step s - 1
+      const int64_t half = (int64_t)(step & 1) * a.ncoord;
+      a.cvec[(a.ncoord - half) + a.coord[slot]] = (T)0;
+      a.cvec[half + j] = (T)c;
+    }
     a.x[j] = xj + d;
     a.coord[slot] = (int32_t)j;
     a.coef[slot] = c;
@@ -186,6 +199,87 @@ __global__ __launch_bounds__(kBlock) void k_step_scatter(StepArgs<T> a) {
     for (int u = 0; u < 4; ++u) unsafeAtomicAdd(&a.res[r[u]], ct * v[u]);
   }
   for (; p < p1; p += G) unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
+}
+
+// ------------------------------------------------ phase B, streamed form
+// The scatter of a whole step as one pass over the transposed matrix: for
+// every residual index i, res_i += sum over its entries (coordinate j, value
+// a) of c_j a, with c_j the step's coefficient (zero for coordinates not
+// sampled).  Taken when a step covers a large share of the entries (SVM
+// minibatches of thousands of sample rows): every entry is streamed once,
+// coalesced, instead of one random atomic per sampled entry.
+//
+// The host cuts the transposed entries into block ranges at residual-index
+// boundaries, so block b owns the indices [f0, f1) outright.  The block
+// copies the step's ncoord coefficients into LDS, adds every entry's
+// c_j a into an LDS accumulator of its f1 - f0 indices (LDS atomics: no
+// global memory operation inside the stream besides the entry loads, which
+// stay U per thread in flight), and ends with one coalesced read-modify-write
+// of res[f0, f1).
+template <typename T>
+struct TEntry {
+  uint16_t lseg;   // residual index - f0 of the entry's block
+  uint16_t coord;  // coordinate (column, or sample row)
+  T val;
+};
+
+template <typename T>
+struct TScatterArgs {
+  const TEntry<T> *ent;
+  const int32_t *eb;  // entries of block b: [eb[b], eb[b+1])
+  const int32_t *fb;  // residual indices of block b: [fb[b], fb[b+1])
+  const T *cvec;      // the step's half: ncoord coefficients
+  int32_t ncoord;
+  T *res;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_step_scatter_t(TScatterArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char tl_raw[];
+  T *cl = reinterpret_cast<T *>(tl_raw);  // coefficients, then the accumulator
+  const int32_t e0 = a.eb[blockIdx.x], e1 = a.eb[blockIdx.x + 1];
+  const int32_t f0 = a.fb[blockIdx.x], f1 = a.fb[blockIdx.x + 1];
+  T *acc = cl + a.ncoord;
+  constexpr int SU = 16;  // loads in flight per thread before their uses
+  for (int32_t i0 = threadIdx.x; i0 < a.ncoord; i0 += SU * kBlock) {
+    T v[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const int32_t i = i0 + u * kBlock;
+      v[u] = i < a.ncoord ? a.cvec[i] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const int32_t i = i0 + u * kBlock;
+      if (i < a.ncoord) cl[i] = v[u];
+    }
+  }
+  for (int32_t k = threadIdx.x; k < f1 - f0; k += kBlock) acc[k] = (T)0;
+  __syncthreads();
+  for (int32_t q0 = e0 + threadIdx.x; q0 < e1; q0 += SU * kBlock) {
+    TEntry<T> e[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) e[u] = a.ent[min((int64_t)q0 + u * kBlock, (int64_t)e1 - 1)];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const T c = cl[e[u].coord];
+      if ((int64_t)q0 + u * kBlock < e1 && c != (T)0) atomicAdd(&acc[e[u].lseg], c * e[u].val);
+    }
+  }
+  __syncthreads();
+  for (int32_t k0 = threadIdx.x; k0 < f1 - f0; k0 += SU * kBlock) {
+    T r[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const int32_t k = k0 + u * kBlock;
+      r[u] = k < f1 - f0 ? a.res[f0 + k] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const int32_t k = k0 + u * kBlock;
+      if (k < f1 - f0 && acc[k] != (T)0) a.res[f0 + k] = r[u] + acc[k];
+    }
+  }
 }
 
 // ------------------------------------------------------ grid-wide barrier

@@ -12,6 +12,8 @@
 //   r64 double[m] FP64 path: same buffer as res; FP32 path: anchor Ax - b
 //   gref double[n] FP32 path: A^T r64 at the last refresh
 //   coord int32[tau], coef double[tau]  staged between the two step phases
+//   streamed scatter only: tent TEntry<T>[nnz] (the transposed copy, by
+//   residual index) and cvec T[2n] (the step coefficients by coordinate)
 // T = double on the FP64 path, float on the FP32+refresh path.
 #include <hip/hip_runtime.h>
 
@@ -67,6 +69,10 @@ constexpr uint64_t kAsyncPerLaunch = 1024;
 constexpr int kAsyncChunks = 4;
 // segment entries per lane kept in registers by the persistent kernel
 constexpr int kRegChunks = 2;
+// dynamic LDS of the streamed scatter: the step's n coefficients (at most
+// kStreamCoefBytes) and a block's accumulator
+constexpr size_t kStreamLdsBytes = 160 * 1024 - 1024;
+constexpr size_t kStreamCoefBytes = 96 * 1024;
 }  // namespace
 
 
@@ -186,6 +192,13 @@ struct acdc_solver {
   bool fp32 = false;
   bool persistent = false;
   bool async = false;  // ACDC_STEP_ASYNC
+  // per-step launches whose scatter streams the transposed matrix
+  // (k_step_scatter_t): its entries, the two coefficient halves, the grid
+  bool streamed = false;
+  void *tent = nullptr, *cvec = nullptr;
+  int32_t *teb = nullptr, *tfb = nullptr;  // block ranges: entries, residual indices
+  unsigned tgrid = 0;
+  size_t tlds = 0;  // dynamic LDS of the streamed scatter
   // asynchronous mode: per-coordinate records and interleaved entries; the
   // records hold x while workers run, the plain x array between runs
   void *rec = nullptr, *ent = nullptr;
@@ -257,6 +270,8 @@ StepArgs<T> step_args(const acdc_solver *s) {
   a.tau = s->set.tau;
   a.coord = s->coord;
   a.coef = s->coef;
+  a.cvec = s->streamed ? static_cast<T *>(s->cvec) : nullptr;
+  a.ncoord = s->ncoord;
   return a;
 }
 
@@ -319,7 +334,18 @@ struct LaunchStep {
     const unsigned blocks = (unsigned)div_up(s->set.tau * G, kBlock);
     const StepArgs<T> a = step_args<T>(s);
     if (part & 1) k_step_gather<T, G, LOSS><<<blocks, kBlock, 0, s->stream>>>(a, step);
-    if (part & 2) k_step_scatter<T, G><<<blocks, kBlock, 0, s->stream>>>(a);
+    if ((part & 2) && s->streamed) {
+      TScatterArgs<T> t;
+      t.ent = static_cast<const TEntry<T> *>(s->tent);
+      t.eb = s->teb;
+      t.fb = s->tfb;
+      t.cvec = a.cvec + (int64_t)(step & 1) * s->ncoord;
+      t.ncoord = (int32_t)s->ncoord;
+      t.res = a.res;
+      k_step_scatter_t<T><<<s->tgrid, kBlock, s->tlds, s->stream>>>(t);
+    } else if (part & 2) {
+      k_step_scatter<T, G><<<blocks, kBlock, 0, s->stream>>>(a);
+    }
   }
 };
 
@@ -597,8 +623,16 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown precision");
   if (set->step_mode != ACDC_STEP_SYNC && set->step_mode != ACDC_STEP_ASYNC)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown step mode");
-  if (set->launch_mode < ACDC_LAUNCH_AUTO || set->launch_mode > ACDC_LAUNCH_PERSISTENT)
+  if (set->launch_mode < ACDC_LAUNCH_AUTO || set->launch_mode > ACDC_LAUNCH_PER_STEP_STREAMED)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown launch mode");
+  const size_t coef_bytes = (size_t)p->ncoord * (set->precision == ACDC_PREC_FP32_REFRESH ? 4 : 8);
+  const bool stream_fits = coef_bytes <= kStreamCoefBytes && p->ncoord <= 65535;
+  if (set->launch_mode == ACDC_LAUNCH_PER_STEP_STREAMED &&
+      (set->step_mode != ACDC_STEP_SYNC || set->tau < 2 || !stream_fits))
+    return fail(ACDC_ERR_INVALID,
+                "acdc_solver_create: the streamed scatter needs synchronous steps with tau >= 2 "
+                "and n <= 65535 coefficients within " + std::to_string(kStreamCoefBytes) +
+                    " bytes of LDS");
   if (set->lanes_per_coord != 0 && set->lanes_per_coord != 8 && set->lanes_per_coord != 16 &&
       set->lanes_per_coord != 32 && set->lanes_per_coord != 64 && set->lanes_per_coord != kBlock)
     return fail(ACDC_ERR_INVALID,
@@ -711,7 +745,9 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
   s->async = set->step_mode == ACDC_STEP_ASYNC;
   for (size_t c = 0; c + 1 < p->cptr.size(); ++c)
     s->maxlen = std::max<int64_t>(s->maxlen, (int64_t)p->cptr[c + 1] - p->cptr[c]);
-  if (!s->async && set->tau >= 2 && set->launch_mode != ACDC_LAUNCH_PER_STEP && s->lanes != kBlock) {
+  if (!s->async && set->tau >= 2 &&
+      (set->launch_mode == ACDC_LAUNCH_AUTO || set->launch_mode == ACDC_LAUNCH_PERSISTENT) &&
+      s->lanes != kBlock) {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     (void)hipGetDevice(&dev);
@@ -732,6 +768,80 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
       return bail(fail(ACDC_ERR_INVALID,
                        "acdc_solver_create: tau * lanes_per_coord exceeds the co-resident grid "
                        "of the persistent kernel"));
+  }
+  // streamed scatter: forced, or automatic when a step holds at least a
+  // tenth of the coordinates (measured crossover on the 20,000-sample SVM:
+  // tau ~1,800; one coalesced pass over every entry against one atomic per
+  // sampled entry)
+  s->streamed = !s->async && !s->persistent && set->tau >= 2 && stream_fits &&
+                (set->launch_mode == ACDC_LAUNCH_PER_STEP_STREAMED ||
+                 (set->launch_mode == ACDC_LAUNCH_AUTO && 10 * set->tau >= s->ncoord));
+  if (s->streamed) {
+    int dev = 0, cus = 1;
+    hipError_t te = hipGetDevice(&dev);
+    if (te == hipSuccess) te = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // block ranges at residual-index boundaries: about nnz / CUs entries
+    // each, at most `cap` indices (the accumulator's LDS, and 16-bit local
+    // indices); one index with more entries is a block of its own
+    const size_t tsz_ = s->fp32 ? 4 : 8;
+    const int64_t cap = std::min<int64_t>(65536, (int64_t)((kStreamLdsBytes - coef_bytes) / tsz_));
+    const int64_t target = std::max<int64_t>(1024, div_up(p->nnz, std::max(1, cus)));
+    std::vector<int64_t> at(s->nres + 1, 0);  // entries per index, then starts
+    for (int32_t i : p->cidx) ++at[i + 1];
+    std::vector<int32_t> eb{0}, fb{0};
+    int64_t span = 0, ents = 0, maxspan = 1;
+    for (int64_t i = 0; i < s->nres; ++i) {
+      const int64_t c = at[i + 1];
+      if (span > 0 && (ents + c > target || span + 1 > cap)) {
+        eb.push_back(eb.back() + (int32_t)ents);
+        fb.push_back((int32_t)i);
+        maxspan = std::max(maxspan, span);
+        span = ents = 0;
+      }
+      ++span;
+      ents += c;
+    }
+    eb.push_back(eb.back() + (int32_t)ents);
+    fb.push_back((int32_t)s->nres);
+    maxspan = std::max(maxspan, span);
+    for (int64_t i = 0; i < s->nres; ++i) at[i + 1] += at[i];
+    const size_t nb = eb.size() - 1;
+    const size_t tesz = s->fp32 ? sizeof(TEntry<float>) : sizeof(TEntry<double>);
+    if ((st = dalloc(s, &s->tent, (size_t)p->nnz * tesz)) != ACDC_OK) return bail(st);
+    if ((st = dalloc(s, &s->cvec, 2 * coef_bytes)) != ACDC_OK) return bail(st);
+    if ((st = dalloc_t(s, &s->teb, nb + 1)) != ACDC_OK) return bail(st);
+    if ((st = dalloc_t(s, &s->tfb, nb + 1)) != ACDC_OK) return bail(st);
+    // entries by residual index (coordinates ascending within one index),
+    // the index stored relative to its block's first
+    std::vector<int32_t> blk_of((size_t)s->nres);
+    for (size_t b = 0; b < nb; ++b)
+      for (int32_t i = fb[b]; i < fb[b + 1]; ++i) blk_of[i] = (int32_t)b;
+    auto fill = [&](auto tag) {
+      using T = decltype(tag);
+      std::vector<TEntry<T>> tv((size_t)p->nnz);
+      for (int64_t j = 0; j < s->ncoord; ++j)
+        for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) {
+          const int32_t i = p->cidx[q];
+          tv[at[i]++] = TEntry<T>{(uint16_t)(i - fb[blk_of[i]]), (uint16_t)j, (T)p->cval[q]};
+        }
+      return hipMemcpy(s->tent, tv.data(), tv.size() * sizeof(TEntry<T>), hipMemcpyHostToDevice);
+    };
+    if (te == hipSuccess && p->nnz) te = s->fp32 ? fill(0.0f) : fill(0.0);
+    if (te == hipSuccess) te = hipMemcpy(s->teb, eb.data(), eb.size() * 4, hipMemcpyHostToDevice);
+    if (te == hipSuccess) te = hipMemcpy(s->tfb, fb.data(), fb.size() * 4, hipMemcpyHostToDevice);
+    // both halves start at zero; the first step clears "its previous
+    // coordinates" at slot coordinates 0
+    if (te == hipSuccess) te = hipMemset(s->cvec, 0, 2 * coef_bytes);
+    if (te == hipSuccess) te = hipMemset(s->coord, 0, (size_t)set->tau * 4);
+    s->tlds = coef_bytes + (size_t)maxspan * tsz_;
+    if (te == hipSuccess)
+      te = hipFuncSetAttribute(s->fp32 ? reinterpret_cast<const void *>(&k_step_scatter_t<float>)
+                                       : reinterpret_cast<const void *>(&k_step_scatter_t<double>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tlds);
+    if (te != hipSuccess)
+      return bail(fail(ACDC_ERR_HIP, std::string("acdc_solver_create: streamed scatter setup: ") +
+                                         hipGetErrorString(te)));
+    s->tgrid = (unsigned)nb;
   }
   if (set->beta_mode == ACDC_BETA_SPECTRAL) {
     double lmax = 1.0;

@@ -13,7 +13,8 @@
 //   -r <file>         CSV log file
 // and the HIP knobs (no reference counterpart):
 //   -s <seed>  -p <0 fp64 | 1 fp32+refresh>  -R <refresh epochs>  -L <lanes>
-//   -P <0 auto | 1 per-step launches | 2 persistent grid>  -M <0 sync | 1 async>
+//   -P <0 auto | 1 per-step launches | 2 persistent grid | 3 per step, streamed scatter>
+//   -M <0 sync | 1 async>
 //   -S <0|1>   spectral ESO beta (the power method of mSDCA.cpp:188-286)
 // In the reference, -I falls through into -M (option_console_parser.h:89-93);
 // here each flag sets only its own field.  omega and beta are computed as in

@@ -96,9 +96,21 @@ typedef struct acdc_settings {
 /* Execution of synchronous steps with tau > 1; the arithmetic is the same. */
 typedef enum acdc_launch_mode {
   ACDC_LAUNCH_AUTO = 0,      /* persistent cooperative grid when all tau slots
-                                fit on the device at once, else per step */
+                                fit on the device at once, else per step,
+                                with the streamed scatter when tau >= n/10
+                                and it applies (see below) */
   ACDC_LAUNCH_PER_STEP = 1,  /* one gather + one scatter launch per step */
-  ACDC_LAUNCH_PERSISTENT = 2 /* persistent grid; fails if it does not fit */
+  ACDC_LAUNCH_PERSISTENT = 2, /* persistent grid; fails if it does not fit */
+  ACDC_LAUNCH_PER_STEP_STREAMED = 3 /* per step, the scatter as one coalesced
+                                pass over a transposed copy of the data (each
+                                block owns a range of residual indices and
+                                sums into LDS) instead of one atomic per
+                                sampled entry: for steps that cover a large
+                                share of the entries (SVM minibatches); needs
+                                n <= 65535 and n * sizeof(T) <= 96 KiB (FP32:
+                                n <= 24576, FP64: n <= 12288).  Same
+                                arithmetic as ACDC_LAUNCH_PER_STEP up to the
+                                order of the additions into the residual */
 } acdc_launch_mode;
 
 /* One evaluation, all in FP64.  Counterparts: SquareLoss::getResultLogRow

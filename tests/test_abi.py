@@ -64,6 +64,20 @@ def test_solver_argument_errors(acdc):
     with pytest.raises(acdc.AcdcError, match="block per coordinate"):
         acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=0.1, tau=2, lanes_per_coord=256,
                     launch_mode=acdc.LAUNCH_PERSISTENT)
+    # the streamed scatter: synchronous steps, tau >= 2, n coefficients in LDS
+    with pytest.raises(acdc.AcdcError, match="streamed scatter"):
+        acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=0.1, tau=1,
+                    launch_mode=acdc.LAUNCH_PER_STEP_STREAMED)
+    with pytest.raises(acdc.AcdcError, match="streamed scatter"):
+        acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=0.1, tau=2, step_mode=acdc.STEP_ASYNC,
+                    launch_mode=acdc.LAUNCH_PER_STEP_STREAMED)
+    n = 21_000  # 168 KB of FP64 coefficients
+    wide = acdc.Problem.from_csc(1, n, np.arange(n + 1), np.zeros(n), np.ones(n), np.ones(1))
+    with pytest.raises(acdc.AcdcError, match="streamed scatter"):
+        acdc.Solver(wide, acdc.LOSS_L1_SQUARE, lam=0.1, tau=2,
+                    launch_mode=acdc.LAUNCH_PER_STEP_STREAMED)
+    with pytest.raises(acdc.AcdcError, match="launch mode"):
+        acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=0.1, tau=2, launch_mode=4)
 
 
 def test_batch_argument_errors(acdc):

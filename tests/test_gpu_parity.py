@@ -86,10 +86,11 @@ def test_tau1_trajectory_fp64(acdc, orc, c1):
 
 
 @pytest.mark.parametrize("lanes,launch", [(8, 1), (16, 1), (32, 1), (64, 1), (256, 1),
-                                          (8, 2), (16, 2), (32, 2), (64, 2)])
+                                          (8, 2), (16, 2), (32, 2), (64, 2), (16, 3), (256, 3)])
 def test_sync_tau_fp64_matches_oracle(acdc, orc, c1, lanes, launch):
     """launch 1: one gather + one scatter launch per step; 2: persistent grid;
-    256 lanes: a whole block per coordinate."""
+    3: per step with the streamed scatter over the transposed copy; 256
+    lanes: a whole block per coordinate."""
     tau = 512
     p = c1.problem()
     s = acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=c1.lam, tau=tau, seed=3, lanes_per_coord=lanes,
@@ -239,6 +240,41 @@ def test_svm_matches_oracle(acdc, orc):
         assert rep["gap"] == pytest.approx(e["gap"], rel=1e-9)
         assert rep["primal"] == pytest.approx(e["primal"], rel=tol)
         assert rep["accuracy"] == pytest.approx(e["accuracy"], abs=1.0 / d.n)
+        assert (s.x >= 0).all() and (s.x <= 1).all()
+        s.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_streamed_scatter_svm(acdc, orc, prec):
+    """SVM minibatches with the streamed scatter (automatic at tau >= n/10,
+    forced below) against the oracle's SDCA steps (FP64), and against the
+    atomic per-step scatter on the same stream (FP32: the two differ only in
+    the order of the FP32 additions into w)."""
+    # rows of ~450 entries: automatic lanes give a block per row, which runs
+    # per step (no persistent grid), so tau = 512 >= n/10 streams by itself
+    d = acdc.gen_svm(2000, 50_000, row_median=300.0, row_sigma=0.901, wstar_nnz=1000,
+                     flip_frac=0.1, seed=5)
+    p = d.problem()
+    precision = acdc.PREC_FP64 if prec == "fp64" else acdc.PREC_FP32_REFRESH
+    for tau, launch in ((512, 0), (64, 3)):
+        s = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=tau, seed=12,
+                        precision=precision, refresh_period=0, launch_mode=launch)
+        steps = 3 * d.n // tau
+        s.run_updates(steps * tau)
+        if prec == "fp64":
+            o = orc.SVM(d.n, d.m, d.ptr, d.idx, d.vals, d.rhs, d.lam, beta=s.beta)
+            o.run(seed=12, tau=tau, step0=0, nsteps=steps)
+            assert rel_inf(s.x, o.alpha) < TOL_SYNC
+            assert rel_inf(s.residuals, o.w) < TOL_SYNC
+            assert s.report()["primal"] == pytest.approx(o.eval()["primal"], rel=TOL_SYNC)
+        else:
+            a = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=tau, seed=12,
+                            precision=precision, refresh_period=0,
+                            launch_mode=acdc.LAUNCH_PER_STEP)
+            a.run_updates(steps * tau)
+            assert rel_inf(s.x, a.x) < 1e-4
+            assert s.report()["primal"] == pytest.approx(a.report()["primal"], rel=1e-5)
+            a.close()
         assert (s.x >= 0).all() and (s.x <= 1).all()
         s.close()
 
