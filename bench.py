@@ -605,6 +605,9 @@ def main():
         result["ttd"] = ttd_c4(acdc, args)
 
     if not args.no_cpu:
+        # OpenMP threads spread over the host's cores (read when the OpenMP
+        # runtime starts, with the oracle library below)
+        os.environ.setdefault("OMP_PROC_BIND", "spread")
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_bind as orc
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -624,6 +627,29 @@ def main():
                                             f"epochs) of the same Lasso from x = 0 in {cw:.1f}s, "
                                             "CAS atomics"}
         log(f"[bench] cpu: {done / cw:.4e} upd/s on {threads} threads")
+        # the same solver on one thread (the serial method, beta = 1), a
+        # quarter of the time budget, and the host it ran on
+        o1 = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, d.lam, beta=1.0)
+        t0 = time.perf_counter()
+        done1, per1 = 0, max(1, n // 20)
+        while time.perf_counter() - t0 < args.cpu_seconds / 4 and done1 < 30 * n:
+            o1.run_async(seed=2, k0=done1, per_worker=per1, threads=1)
+            done1 += per1
+        cw1 = time.perf_counter() - t0
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f
+                              if ln.startswith("model name")), "")
+        except OSError:
+            pass
+        result["cpu_baseline"].update({
+            "one_thread": {"value": done1 / cw1, "unit": "coord updates/s",
+                           "sample": f"{done1} serial FP64 updates in {cw1:.1f}s"},
+            "host": {"cpu_model": model, "nproc": os.cpu_count(),
+                     "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+                     "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")}})
+        log(f"[bench] cpu: {done1 / cw1:.4e} upd/s on 1 thread ({model})")
     print(json.dumps(result), flush=True)
 
 
