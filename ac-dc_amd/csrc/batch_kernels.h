@@ -136,7 +136,7 @@ struct BatchArgs {
 
 // zeroes x of every coordinate record and the work-queue counters (a kernel
 // rather than a memset so that the whole run can live in one captured graph)
-static __global__ __launch_bounds__(kBlock) void k_batch_reset(CoordRec *__restrict__ cs,
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void k_batch_reset(CoordRec *__restrict__ cs,
                                                                int64_t ncs,
                                                                unsigned int *__restrict__ queues,
                                                                int nq) {

@@ -762,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_gather(Seg<double> A, int64_t nse
 // MulticoreSquareLoss::recomputeResiduals, MulticoreSquareLoss.h:42-53, and of
 // MulticoreSVMDualLoss.h:31-46 with alpha clipped to [0,1]).  Only segments
 // of nonzero coordinates are read, so a sparse x costs O(n + nnz(active)).
-static __global__ __launch_bounds__(kBlock) void k_res_init(int64_t nres, const double *__restrict__ b,
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void k_res_init(int64_t nres, const double *__restrict__ b,
                                                      double *__restrict__ r64) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k < nres) r64[k] = b ? -b[k] : 0.0;
@@ -911,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_coord(Seg<T> A, int64_t nseg,
 }
 
 // Per-residual-entry statistics: {sum r_k^2, sum b_k r_k}.
-static __global__ __launch_bounds__(kBlock) void k_eval_res(int64_t nres, const double *__restrict__ r64,
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void k_eval_res(int64_t nres, const double *__restrict__ r64,
                                                      const double *__restrict__ b,
                                                      double *__restrict__ partial) {
   double v0 = 0, v1 = 0;
@@ -939,7 +939,7 @@ static __global__ __launch_bounds__(kBlock) void k_eval_res(int64_t nres, const 
 
 // Folds nrows partial rows into out[kStats] with one block: thread t folds
 // rows t, t+256, ... in order, then a fixed pairwise tree.
-static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
                                                         int64_t nrows, uint32_t maxmask,
                                                         uint32_t minmask,
                                                         double *__restrict__ out) {
@@ -963,7 +963,7 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__r
 
 // --------------------------------------------------------------- sampling
 // out[s*tau + i] = perm_{seed, step0+s}(i): the device side of the host replay.
-static __global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t seed, uint64_t step0,
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t seed, uint64_t step0,
                                                    uint64_t nsteps, int64_t tau,
                                                    int32_t *__restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -974,7 +974,7 @@ static __global__ __launch_bounds__(kBlock) void k_sample(Perm perm, uint64_t se
 }
 
 // out[s*tau + w] = coordinate of update step0+s of asynchronous worker w.
-static __global__ __launch_bounds__(kBlock) void k_sample_async(uint32_t n, uint64_t seed,
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void k_sample_async(uint32_t n, uint64_t seed,
                                                          uint64_t step0, uint64_t nsteps,
                                                          int64_t tau, int32_t *__restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
