@@ -330,7 +330,9 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
   } else {
     // many nonzeros: the statistics over the records, then every entry
     // streamed (wave-contiguous ranges, the next round's loads in flight)
-    // with its column's x gathered beside it
+    // with its column's x gathered beside it.  SVM: the column factor
+    // alpha_j y_j / (lambda n) is parked in gref (which the gradient pass
+    // below rewrites) and gathered instead of x.
     constexpr int XU = 8;
     for (int j0 = tid; j0 < I.n; j0 += XU * kBatchBlock) {
       double xs[XU];
@@ -344,7 +346,15 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
         x1 += fabs(xs[u]);
         nz += xs[u] != 0.0;
       }
+      if constexpr (svm)
+#pragma unroll
+        for (int u = 0; u < XU; ++u) {
+          const int j = j0 + u * kBatchBlock;
+          if (j < I.n)
+            *gp(&I.cs[j].gref) = xs[u] != 0.0 ? xs[u] * sign_of(gld(&I.cs[j].L)) * inv_ln : 0.0;
+        }
     }
+    if constexpr (svm) __syncthreads();  // the factors are in before the stream reads them
     if (e_lo < e_hi) {
       RefEntry nxt[U];
       auto load = [&](int32_t base) {
@@ -359,10 +369,8 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
         load(base + 64 * U);
         double xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = gld_agent(&I.cs[cur[u].col].x);
-        if (svm)
-#pragma unroll
-          for (int u = 0; u < U; ++u) xv[u] *= sign_of(gld(&I.cs[cur[u].col].L)) * inv_ln;
+        for (int u = 0; u < U; ++u)
+          xv[u] = gld_agent(svm ? &I.cs[cur[u].col].gref : &I.cs[cur[u].col].x);
 #pragma unroll
         for (int u = 0; u < U; ++u)
           if (base + u * 64 + ln < e_hi && xv[u] != 0.0) radd(cur[u].row, cur[u].val * xv[u]);
