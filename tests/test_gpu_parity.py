@@ -312,6 +312,38 @@ def test_edge_cases(acdc, orc):
     s.close()
 
 
+@pytest.mark.parametrize("loss", [0, 10])
+def test_streamed_scatter_edge_cases(acdc, orc, loss):
+    """The streamed scatter's block ranges: a row in every column (more
+    entries than a block's target: a block of its own), a run of empty rows,
+    empty columns; FP64 against the oracle's synchronous steps (Lasso, NNLS)."""
+    rng = np.random.default_rng(31)
+    m, n, k = 3000, 1500, 5
+    cols = []
+    for j in range(n):
+        if j % 97 == 5:
+            cols.append(np.array([], dtype=np.int64))  # empty column
+            continue
+        rows = 100 + rng.choice(m - 100, k, replace=False)  # rows 1..99 stay empty
+        cols.append(np.sort(np.concatenate([[0], rows])))
+    ptr = np.concatenate([[0], np.cumsum([c.size for c in cols])]).astype(np.int32)
+    idx = np.concatenate(cols).astype(np.int32)
+    vals = rng.standard_normal(idx.size)
+    b = rng.standard_normal(m)
+    p = acdc.Problem.from_csc(m, n, ptr, idx, vals, b)
+    lam = 0.05 if loss == 0 else 0.0
+    tau = 256
+    s = acdc.Solver(p, loss, lam=lam, tau=tau, seed=8, launch_mode=acdc.LAUNCH_PER_STEP_STREAMED)
+    o = orc.CD(loss, m, n, ptr, idx, vals, b, lam, beta=s.beta)
+    steps = 3 * (-(-n // tau))
+    s.run_updates(steps * tau)
+    o.run(seed=8, tau=tau, step0=0, nsteps=steps)
+    assert rel_inf(s.x, o.x) < TOL_SYNC
+    assert rel_inf(s.residuals, o.r) < TOL_SYNC
+    assert s.report()["primal"] == pytest.approx(o.eval()["primal"], rel=TOL_SYNC)
+    s.close()
+
+
 def test_warm_start_and_lambda_path(acdc, orc, c1):
     p = c1.problem()
     s = acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=c1.lambda_max * 0.5, tau=256, seed=4)
