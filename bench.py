@@ -384,16 +384,19 @@ def svm_c3(acdc, args):
     """Hinge-SVM dual minibatch SDCA on 20,000 samples x 1,350,000 features
     (lognormal row nnz, median 300): synchronous minibatches b in {512, 2048,
     8192} (tau = b, ESO beta of b), FP32 + FP64 refresh of w every 5 epochs
-    and the plain FP64 path; updates/s over the first 20 epochs and the FP64
-    relative duality gap after them."""
+    and the plain FP64 path, and b asynchronous FP32 workers (the multicore
+    executor's semantics, same beta); updates/s over the first 20 epochs and
+    the FP64 relative duality gap after them."""
     d = acdc.config_svm_c3()
     p = d.problem()
     out = {"workload": f"svm dual {d.n} samples x {d.m} features", "nnz": d.nnz,
            "epochs": 20, "rows": []}
     for b in (512, 2048, 8192):
-        for prec, name in ((acdc.PREC_FP32_REFRESH, "fp32"), (acdc.PREC_FP64, "fp64")):
+        for prec, mode, name in ((acdc.PREC_FP32_REFRESH, acdc.STEP_SYNC, "fp32"),
+                                 (acdc.PREC_FP64, acdc.STEP_SYNC, "fp64"),
+                                 (acdc.PREC_FP32_REFRESH, acdc.STEP_ASYNC, "fp32 async")):
             s = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=b, seed=3,
-                            precision=prec, refresh_period=5)
+                            precision=prec, refresh_period=5, step_mode=mode)
             s.synchronize()
             t0 = time.perf_counter()
             s.run_updates(20 * d.n)
