@@ -492,10 +492,11 @@ def main():
     mean_nnz = d.nnz / d.n
     maxlen = int(np.diff(d.ptr).max())
 
-    def make(precision, mode, launch=acdc.LAUNCH_AUTO, refresh=5, lanes=0):
+    def make(precision, mode, launch=acdc.LAUNCH_AUTO, refresh=5, lanes=0,
+             beta_mode=acdc.BETA_OMEGA):
         return acdc.Solver(prob, acdc.LOSS_L1_SQUARE, lam=d.lam, tau=tau, seed=2,
                            precision=precision, refresh_period=refresh, step_mode=mode,
-                           launch_mode=launch, lanes_per_coord=lanes)
+                           launch_mode=launch, lanes_per_coord=lanes, beta_mode=beta_mode)
 
     # ---- lanes per coordinate: the autotuner unless given
     if args.lanes == 0:
@@ -597,6 +598,25 @@ def main():
         if result["time_to_gap"]["reached"] and result["fp64_plain"]["time_to_gap"]["reached"]:
             result["time_to_gap"]["speedup_vs_fp64_plain"] = (
                 result["fp64_plain"]["time_to_gap"]["seconds"] / result["time_to_gap"]["seconds"])
+        # the same two paths with the spectral ESO beta of mSDCA.cpp (20 power
+        # steps on the normalised Gram matrix) instead of the configured
+        # omega rule: a side result, both paths with the same beta
+        spec = {}
+        for name, prec, mode, launch, lanes in (
+                ("fp32_async", acdc.PREC_FP32_REFRESH, acdc.STEP_ASYNC, acdc.LAUNCH_AUTO, args.lanes),
+                ("fp64_plain", acdc.PREC_FP64, acdc.STEP_SYNC, acdc.LAUNCH_PER_STEP, 0)):
+            sp = make(prec, mode, launch=launch, lanes=lanes, beta_mode=acdc.BETA_SPECTRAL)
+            tt, ep, rep = time_to_gap(sp, n, args.gap_target, args.gap_every, args.max_epochs)
+            spec[name] = {"seconds": tt, "epochs": ep, "rel_gap": rep["rel_gap"],
+                          "reached": rep["rel_gap"] <= args.gap_target}
+            spec["beta"] = sp.beta
+            log(f"[bench] spectral beta {sp.beta:.4f} {name}: rel gap {rep['rel_gap']:.3e} "
+                f"after {ep} epochs in {tt:.3f}s")
+            sp.close()
+        if spec["fp32_async"]["reached"] and spec["fp64_plain"]["reached"]:
+            spec["speedup_vs_fp64_plain"] = (spec["fp64_plain"]["seconds"] /
+                                             spec["fp32_async"]["seconds"])
+        result["time_to_gap_spectral_beta"] = spec
     s64.close()
 
     if not args.no_batch:
