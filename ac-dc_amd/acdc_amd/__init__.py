@@ -101,7 +101,7 @@ class BatchSettings(ctypes.Structure):
                 ("max_epochs", ctypes.c_int32), ("gap_tol", ctypes.c_double),
                 ("use_graph", ctypes.c_int32), ("step_mode", ctypes.c_int32),
                 ("tau", ctypes.c_int32), ("queue_chunk", ctypes.c_int32),
-                ("loss", ctypes.c_int32)]
+                ("loss", ctypes.c_int32), ("beta_mode", ctypes.c_int32)]
 
 
 class _BatchResult(ctypes.Structure):
@@ -404,7 +404,7 @@ class Batch:
     def __init__(self, problems, lambdas, *, seed=1, precision=PREC_FP32_REFRESH,
                  lanes_per_coord=0, refresh_period=5, max_epochs=200, gap_tol=1e-6,
                  use_graph=True, step_mode=STEP_SYNC, tau=0, queue_chunk=0,
-                 loss=LOSS_L1_SQUARE):
+                 loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA):
         lam = _f64(lambdas)
         if lam.ndim != 2 or lam.shape[0] != len(problems):
             raise ValueError("lambdas must be (instances, path_len)")
@@ -413,7 +413,7 @@ class Batch:
         self.n = [p.n for p in problems]
         arr = (_P * self.count)(*[p._h for p in problems])
         st = BatchSettings(seed, precision, lanes_per_coord, refresh_period, max_epochs, gap_tol,
-                           int(bool(use_graph)), step_mode, tau, queue_chunk, loss)
+                           int(bool(use_graph)), step_mode, tau, queue_chunk, loss, beta_mode)
         h = _P()
         _check(lib.acdc_batch_create(arr, self.count, _ptr(lam, ctypes.c_double), self.path_len,
                                      ctypes.byref(st), ctypes.byref(h)))
@@ -455,11 +455,11 @@ class Batch:
 
 def batch_tune(problems, *, seed=1, precision=PREC_FP32_REFRESH, step_mode=STEP_ASYNC,
                refresh_period=5, max_epochs=200, gap_tol=1e-6, use_graph=True,
-               loss=LOSS_L1_SQUARE) -> dict:
+               loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA) -> dict:
     """acdc_batch_tune: lanes per coordinate, tau and queue chunking for a batch."""
     arr = (_P * len(problems))(*[p._h for p in problems])
     base = BatchSettings(seed, precision, 0, refresh_period, max_epochs, gap_tol,
-                         int(bool(use_graph)), step_mode, 0, 0, loss)
+                         int(bool(use_graph)), step_mode, 0, 0, loss, beta_mode)
     out = BatchSettings()
     score = ctypes.c_double()
     _check(lib.acdc_batch_tune(arr, len(problems), ctypes.byref(base), ctypes.byref(out),

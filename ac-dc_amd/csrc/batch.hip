@@ -156,6 +156,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   if (set->lanes_per_coord != 0 && set->lanes_per_coord != 8 && set->lanes_per_coord != 16 &&
       set->lanes_per_coord != 32 && set->lanes_per_coord != 64)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: lanes_per_coord must be 0, 8, 16, 32 or 64");
+  if (set->beta_mode != ACDC_BETA_OMEGA && set->beta_mode != ACDC_BETA_SPECTRAL)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: beta_mode must be omega or spectral");
   if (set->loss != ACDC_LOSS_L1_SQUARE && set->loss != ACDC_LOSS_NNLS &&
       set->loss != ACDC_LOSS_SVM_DUAL_HINGE)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: loss must be L1-square, NNLS or SVM dual");
@@ -281,9 +283,25 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     I.n = (int32_t)p->ncoord;
     I.tau = (int32_t)std::min<int64_t>(slots, p->ncoord);
     I.seed = set->seed + (uint64_t)i;
-    const double beta = p->ncoord > 1 ? 1.0 + (double)(p->omega - 1) * (double)(I.tau - 1) /
-                                                  ((double)p->ncoord - 1.0)
-                                      : 1.0;
+    double beta = p->ncoord > 1 ? 1.0 + (double)(p->omega - 1) * (double)(I.tau - 1) /
+                                            ((double)p->ncoord - 1.0)
+                                : 1.0;
+    if (set->beta_mode == ACDC_BETA_SPECTRAL) {
+      // the single solver's power method on this instance (FP64, same tau)
+      acdc_settings ss{};
+      ss.lambda = svm ? lambdas[(int64_t)i * path_len] : 0.0;
+      ss.tau = I.tau;
+      ss.seed = I.seed;
+      ss.beta_mode = ACDC_BETA_SPECTRAL;
+      ss.precision = ACDC_PREC_FP64;
+      ss.step_mode = ACDC_STEP_SYNC;
+      ss.launch_mode = ACDC_LAUNCH_PER_STEP;
+      acdc_solver *sv = nullptr;
+      acdc_status bs = acdc_solver_create(p, set->loss, &ss, &sv);
+      if (bs == ACDC_OK) bs = acdc_get_beta(sv, &beta);
+      if (sv) acdc_solver_destroy(sv);
+      if (bs != ACDC_OK) return bail(bs);
+    }
     b->n.push_back(p->ncoord);
     b->m.push_back(p->nres);
     b->xoff.push_back(on);

@@ -269,6 +269,11 @@ typedef struct acdc_batch_settings {
                               (problems from acdc_problem_create_csr, lambdas
                               > 0; x is alpha; primal and dual as
                               acdc_report's; nnz_x counts alpha_j != 0) */
+  int32_t beta_mode;       /* ACDC_BETA_OMEGA (0: 1 + (omega-1)(tau-1)/(n-1)
+                              per instance) or ACDC_BETA_SPECTRAL (per
+                              instance, from 20 power steps on its normalised
+                              Gram matrix at create, as acdc_solver_create
+                              does: mSDCA.cpp:188-286) */
 } acdc_batch_settings;
 
 typedef struct acdc_batch_result {

@@ -97,6 +97,8 @@ def test_batch_argument_errors(acdc):
         acdc.Batch([csc], np.array([[0.1, -1.0]]))
     with pytest.raises(acdc.AcdcError, match="lanes_per_coord"):
         acdc.Batch([csc], lam, lanes_per_coord=12)
+    with pytest.raises(acdc.AcdcError, match="beta_mode"):
+        acdc.Batch([csc], lam, beta_mode=acdc.BETA_FORCED)
     with pytest.raises(acdc.AcdcError, match="SVM problems must be sample rows"):
         acdc.batch_tune([csc], loss=acdc.LOSS_SVM_DUAL_HINGE)
 

@@ -368,3 +368,28 @@ def test_batch_svm_autotuner(acdc, svm_batch):
     assert t["lanes_per_coord"] in (8, 16, 32)
     assert 1 <= t["tau"] <= 1024 // t["lanes_per_coord"]
     assert t["score"] > 0
+
+
+def test_batch_spectral_beta(acdc, orc, small_batch):
+    """beta_mode = spectral: each instance's beta is the single solver's
+    spectral beta at the instance's tau and seed, and the synchronous FP64
+    steps with it follow the oracle's."""
+    ds = small_batch[:3]
+    lams = lambdas_of(acdc, ds, 2, 0.1)
+    probs = [d.problem() for d in ds]
+    b = acdc.Batch(probs, lams, seed=44, precision=acdc.PREC_FP64, lanes_per_coord=16,
+                   refresh_period=1, max_epochs=2, gap_tol=-1.0, use_graph=False,
+                   beta_mode=acdc.BETA_SPECTRAL)
+    b.run()
+    for i, d in enumerate(ds):
+        tau, beta, _ = b.info(i)
+        s = acdc.Solver(probs[i], acdc.LOSS_L1_SQUARE, lam=0.0, tau=tau, seed=44 + i,
+                        beta_mode=acdc.BETA_SPECTRAL)
+        assert beta == pytest.approx(s.beta, rel=1e-12)  # atomic sums in the power method
+        s.close()
+        omega_beta = orc.eso_beta(orc.omega(d.m, d.idx), tau, d.n)
+        assert 1.0 <= beta <= omega_beta + 1e-12
+        o = oracle_path(orc, d, lams[i], 44 + i, tau, beta, 2)
+        x = b.x(i)
+        assert np.max(np.abs(x - o.x)) <= 1e-10 * max(1.0, np.max(np.abs(o.x))), i
+    b.close()
