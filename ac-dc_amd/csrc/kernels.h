@@ -579,10 +579,17 @@ __global__ __launch_bounds__(kAsyncBlock) void k_async(AsyncArgs<T> a, uint64_t 
     T acc = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) acc += cur.cv[r] * v[r];
-    if (TAIL)
-      for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
-        const Entry<T> e = a.ent[cur.r.p0 + q];
-        acc += e.val * ld_coherent(&a.res[e.row]);
+    if (TAIL)  // entries past the cached ones, four chunks per round trip
+      for (int32_t q0 = lane + R * G; q0 < cur.r.len; q0 += 4 * G) {
+        Entry<T> e[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = a.ent[cur.r.p0 + min(q0 + u * G, cur.r.len - 1)];
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ld_coherent(&a.res[e[u].row]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (q0 + u * G < cur.r.len) acc += e[u].val * v[u];
       }
     acc = group_sum<G>(acc);
     double c;
@@ -625,9 +632,13 @@ __global__ __launch_bounds__(kAsyncBlock) void k_async(AsyncArgs<T> a, uint64_t 
       for (int r = 0; r < R; ++r)
         if (lane + r * G < cur.r.len) unsafeAtomicAdd(&a.res[cur.ci[r]], ct * cur.cv[r]);
       if (TAIL)
-        for (int32_t q = lane + R * G; q < cur.r.len; q += G) {
-          const Entry<T> e = a.ent[cur.r.p0 + q];
-          unsafeAtomicAdd(&a.res[e.row], ct * e.val);
+        for (int32_t q0 = lane + R * G; q0 < cur.r.len; q0 += 4 * G) {
+          Entry<T> e[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) e[u] = a.ent[cur.r.p0 + min(q0 + u * G, cur.r.len - 1)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (q0 + u * G < cur.r.len) unsafeAtomicAdd(&a.res[e[u].row], ct * e[u].val);
         }
     }
     if (serial) wait_memory();
