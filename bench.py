@@ -258,7 +258,8 @@ def batched_plain_sample(acdc, args, ds, probs, lams, loss, kw, sample, fixed_ep
         tau = int(bi.info(0)[0])
         bi.close()
         s = acdc.Solver(p, loss, lam=float(lams[i, 0]), tau=tau, seed=5,
-                        precision=acdc.PREC_FP64, launch_mode=acdc.LAUNCH_PER_STEP)
+                        precision=acdc.PREC_FP64, launch_mode=acdc.LAUNCH_PER_STEP,
+                        beta_mode=kw.get("beta_mode", acdc.BETA_OMEGA))
         s.synchronize()
         t0 = time.perf_counter()
         for l in range(P):
@@ -302,8 +303,9 @@ def batched_svm(acdc, args):
     lambda = 10/n -> 1/n, stopping at FP64 relative duality gap <= the gap
     target or 100 epochs per lambda; one graph-captured launch (FP32 + FP64
     refresh of w every 5 epochs, asynchronous workers with compare-and-swap
-    steps on [0, 1]); beside it the plain FP64 path on three sampled
-    instances over the same paths."""
+    steps on [0, 1], the spectral ESO beta of mSDCA.cpp per instance); beside
+    it the plain FP64 path (same beta rule) on three sampled instances over
+    the same paths."""
     import numpy as np
     t = time.perf_counter()
     rng = np.random.default_rng(6)
@@ -319,11 +321,13 @@ def batched_svm(acdc, args):
         f"{time.perf_counter() - t:.1f}s")
     probs = [d.problem() for d in ds]
     lams = np.stack([np.geomspace(10.0, 1.0, 8) / d.n for d in ds])
-    tuned = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC, loss=acdc.LOSS_SVM_DUAL_HINGE)
+    tuned = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC, loss=acdc.LOSS_SVM_DUAL_HINGE,
+                            beta_mode=acdc.BETA_SPECTRAL)
     kw = dict(seed=5, precision=acdc.PREC_FP32_REFRESH, refresh_period=5, max_epochs=100,
               gap_tol=args.gap_target, step_mode=acdc.STEP_ASYNC,
               lanes_per_coord=tuned["lanes_per_coord"], tau=tuned["tau"],
-              queue_chunk=tuned["queue_chunk"], loss=acdc.LOSS_SVM_DUAL_HINGE)
+              queue_chunk=tuned["queue_chunk"], loss=acdc.LOSS_SVM_DUAL_HINGE,
+              beta_mode=acdc.BETA_SPECTRAL)
     b = acdc.Batch(probs, lams, use_graph=True, **kw)
     t0 = time.perf_counter()
     ms = b.run()
@@ -390,20 +394,23 @@ def batched_ttd(acdc, args):
 def svm_c3(acdc, args):
     """Hinge-SVM dual minibatch SDCA on 20,000 samples x 1,350,000 features
     (lognormal row nnz, median 300): synchronous minibatches b in {512, 2048,
-    8192} (tau = b, ESO beta of b), FP32 + FP64 refresh of w every 5 epochs
-    and the plain FP64 path, and b asynchronous FP32 workers (the multicore
-    executor's semantics, same beta); updates/s over the first 20 epochs and
-    the FP64 relative duality gap after them."""
+    8192} (tau = b, the spectral ESO beta of mSDCA.cpp:188-286, the rule of
+    the reference's minibatch SDCA experiments), FP32 + FP64 refresh of w
+    every 5 epochs and the plain FP64 path, and b asynchronous FP32 workers
+    (the multicore executor's semantics and its omega beta,
+    MultiCoreSolver.cpp:55-62, 108-110); updates/s over the first 20 epochs
+    and the FP64 relative duality gap after them."""
     d = acdc.config_svm_c3()
     p = d.problem()
     out = {"workload": f"svm dual {d.n} samples x {d.m} features", "nnz": d.nnz,
            "epochs": 20, "rows": []}
     for b in (512, 2048, 8192):
-        for prec, mode, name in ((acdc.PREC_FP32_REFRESH, acdc.STEP_SYNC, "fp32"),
-                                 (acdc.PREC_FP64, acdc.STEP_SYNC, "fp64"),
-                                 (acdc.PREC_FP32_REFRESH, acdc.STEP_ASYNC, "fp32 async")):
+        for prec, mode, bm, name in (
+                (acdc.PREC_FP32_REFRESH, acdc.STEP_SYNC, acdc.BETA_SPECTRAL, "fp32"),
+                (acdc.PREC_FP64, acdc.STEP_SYNC, acdc.BETA_SPECTRAL, "fp64"),
+                (acdc.PREC_FP32_REFRESH, acdc.STEP_ASYNC, acdc.BETA_OMEGA, "fp32 async")):
             s = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=b, seed=3,
-                            precision=prec, refresh_period=5, step_mode=mode)
+                            precision=prec, refresh_period=5, step_mode=mode, beta_mode=bm)
             s.synchronize()
             t0 = time.perf_counter()
             s.run_updates(20 * d.n)
@@ -411,6 +418,7 @@ def svm_c3(acdc, args):
             wall = time.perf_counter() - t0
             r = s.report()
             out["rows"].append({"b": b, "precision": name, "beta": s.beta,
+                                "beta_rule": "spectral" if bm == acdc.BETA_SPECTRAL else "omega",
                                 "updates_per_s": 20 * d.n / wall, "rel_gap": r["rel_gap"],
                                 "accuracy": r["accuracy"]})
             log(f"[bench] svm b={b} {name}: {20 * d.n / wall:.3e} upd/s, rel gap {r['rel_gap']:.2e}")
