@@ -393,3 +393,25 @@ def test_batch_spectral_beta(acdc, orc, small_batch):
         x = b.x(i)
         assert np.max(np.abs(x - o.x)) <= 1e-10 * max(1.0, np.max(np.abs(o.x))), i
     b.close()
+
+
+def test_batch_spectral_beta_svm(acdc, svm_batch):
+    """Spectral beta for SVM instances: the single solver's, at the
+    instance's tau and seed (its power method runs on the sample rows)."""
+    ds = svm_batch[:2]
+    lams = svm_lambdas(ds)
+    probs = [d.problem() for d in ds]
+    b = acdc.Batch(probs, lams, seed=46, precision=acdc.PREC_FP32_REFRESH, refresh_period=5,
+                   max_epochs=100, gap_tol=1e-5, use_graph=False, step_mode=acdc.STEP_ASYNC,
+                   loss=acdc.LOSS_SVM_DUAL_HINGE, beta_mode=acdc.BETA_SPECTRAL)
+    b.run()
+    res = b.results()
+    assert res["converged"].all(), res["rel_gap"]
+    for i, d in enumerate(ds):
+        tau, beta, _ = b.info(i)
+        s = acdc.Solver(probs[i], acdc.LOSS_SVM_DUAL_HINGE, lam=float(lams[i, 0]), tau=tau,
+                        seed=46 + i, beta_mode=acdc.BETA_SPECTRAL)
+        assert beta == pytest.approx(s.beta, rel=1e-12)
+        s.close()
+        assert 1.0 <= beta
+    b.close()
