@@ -799,6 +799,46 @@ __global__ __launch_bounds__(kBlock) void k_refresh_scatter(Seg<T> A, int64_t ns
     unsafeAtomicAdd(&r64[A.idx[p]], (double)A.val[p] * coefx);
 }
 
+// The same r64 as k_refresh_scatter (SVM: w from every alpha), gathered by
+// residual index from an FP64 transposed copy At (segment i: the entries of
+// residual index i, coordinates as indices): r64_i = sum c_j a_ij with the
+// coefficients c of k_svm_coef.  One G-lane group per index, no atomics;
+// taken where nearly every coordinate is nonzero (SVM duals), so that the
+// refresh reads every entry once instead of adding each with an atomic.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_seg_dot(Seg<double> At, int64_t nseg,
+                                                    const double *__restrict__ c,
+                                                    double *__restrict__ out) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t ngroups = (int64_t)gridDim.x * (kBlock / G);
+  for (int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G; i < nseg; i += ngroups) {
+    const int32_t p0 = At.ptr[i], p1 = At.ptr[i + 1];
+    double acc = 0.0;
+    int32_t p = p0 + lane;
+    for (; p + 3 * G < p1; p += 4 * G) {  // four entries in flight, then their gathers
+      int32_t j[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) j[u] = At.idx[p + u * G];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += At.val[p + u * G] * c[j[u]];
+    }
+    for (; p < p1; p += G) acc += At.val[p] * c[At.idx[p]];
+    acc = group_sum<G>(acc);
+    if (lane == 0) out[i] = acc;
+  }
+}
+
+// c_j = clip(alpha_j, 0, 1) y_j / (lambda n): the factor of sample j in w
+// (the SVM branch of k_refresh_scatter)
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void k_svm_coef(
+    int64_t n, const double *__restrict__ x, const double *__restrict__ y, double inv_lambda_n,
+    double *__restrict__ c) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const double a = x[j] > 1.0 ? 1.0 : (x[j] < 0.0 ? 0.0 : x[j]);
+  c[j] = a * y[j] * inv_lambda_n;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(T *__restrict__ v, int64_t n, T value) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;

@@ -197,6 +197,10 @@ struct acdc_solver {
   bool streamed = false;
   void *tent = nullptr, *cvec = nullptr;
   int32_t *teb = nullptr, *tfb = nullptr;  // block ranges: entries, residual indices
+  // SVM: FP64 transposed copy for the refresh's gather of w (k_seg_dot)
+  int32_t *wptr = nullptr, *widx = nullptr;
+  double *wval = nullptr, *wcoef = nullptr;
+  int wlanes = 8;
   unsigned tgrid = 0;
   size_t tlds = 0;  // dynamic LDS of the streamed scatter
   // asynchronous mode: per-coordinate records and interleaved entries; the
@@ -486,8 +490,25 @@ acdc_status refresh(acdc_solver *s) {
   if (st != ACDC_OK) return st;
   if (s->fp32) s->rec_cur = false;  // the anchor changes
   const unsigned rb = (unsigned)div_up(s->nres, kBlock);
-  k_res_init<<<rb, kBlock, 0, s->stream>>>(s->nres, s->loss == kSvm ? nullptr : s->rhs, s->r64);
-  dispatch3<LaunchRefreshScatter>(s, s->eval_lanes, s->r64);
+  if (s->wptr) {  // SVM: w gathered per feature from the transposed copy
+    k_svm_coef<<<(unsigned)div_up(s->ncoord, kBlock), kBlock, 0, s->stream>>>(
+        s->ncoord, s->x, s->rhs, scalars(s).inv_lambda_n, s->wcoef);
+    const Seg<double> wt{s->wptr, s->widx, s->wval};
+    auto go = [&](auto tag) {
+      constexpr int G = decltype(tag)::value;
+      const int64_t blocks = std::min(kEvalBlocks, div_up(s->nres * G, kBlock));
+      k_seg_dot<G><<<(unsigned)blocks, kBlock, 0, s->stream>>>(wt, s->nres, s->wcoef, s->r64);
+    };
+    switch (s->wlanes) {
+      case 8: go(std::integral_constant<int, 8>{}); break;
+      case 16: go(std::integral_constant<int, 16>{}); break;
+      case 32: go(std::integral_constant<int, 32>{}); break;
+      default: go(std::integral_constant<int, 64>{}); break;
+    }
+  } else {
+    k_res_init<<<rb, kBlock, 0, s->stream>>>(s->nres, s->loss == kSvm ? nullptr : s->rhs, s->r64);
+    dispatch3<LaunchRefreshScatter>(s, s->eval_lanes, s->r64);
+  }
   if (s->fp32) {
     dispatch3<LaunchEvalCoord>(s, s->eval_lanes, s->gref);
     k_fill<float><<<rb, kBlock, 0, s->stream>>>(static_cast<float *>(s->res), s->nres, 0.0f);
@@ -777,6 +798,38 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
       return bail(fail(ACDC_ERR_INVALID,
                        "acdc_solver_create: tau * lanes_per_coord exceeds the co-resident grid "
                        "of the persistent kernel"));
+  }
+  // SVM: the FP64 transposed copy of the refresh (w by feature, no atomics);
+  // every alpha_j is usually nonzero, so the scatter form would add all
+  // entries with FP64 atomics
+  if (loss == kSvm) {
+    std::vector<int32_t> wp((size_t)s->nres + 1, 0);
+    for (int32_t i : p->cidx) ++wp[(size_t)i + 1];
+    for (int64_t i = 0; i < s->nres; ++i) wp[i + 1] += wp[i];
+    std::vector<int32_t> wi((size_t)p->nnz);
+    std::vector<double> wv((size_t)p->nnz);
+    {
+      std::vector<int32_t> at(wp.begin(), wp.end() - 1);
+      for (int64_t j = 0; j < s->ncoord; ++j)
+        for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) {
+          const int32_t k = at[p->cidx[q]]++;
+          wi[k] = (int32_t)j;
+          wv[k] = p->cval[q];
+        }
+    }
+    if ((st = dalloc_t(s, &s->wptr, wp.size())) != ACDC_OK) return bail(st);
+    if ((st = dalloc_t(s, &s->widx, wi.size())) != ACDC_OK) return bail(st);
+    if ((st = dalloc_t(s, &s->wval, wv.size())) != ACDC_OK) return bail(st);
+    if ((st = dalloc_t(s, &s->wcoef, (size_t)s->ncoord)) != ACDC_OK) return bail(st);
+    hipError_t we = hipMemcpy(s->wptr, wp.data(), wp.size() * 4, hipMemcpyHostToDevice);
+    if (we == hipSuccess && p->nnz)
+      we = hipMemcpy(s->widx, wi.data(), wi.size() * 4, hipMemcpyHostToDevice);
+    if (we == hipSuccess && p->nnz)
+      we = hipMemcpy(s->wval, wv.data(), wv.size() * 8, hipMemcpyHostToDevice);
+    if (we != hipSuccess)
+      return bail(fail(ACDC_ERR_HIP, std::string("acdc_solver_create: transposed copy: ") +
+                                         hipGetErrorString(we)));
+    s->wlanes = auto_eval_lanes(p->nnz, s->nres);
   }
   // streamed scatter: forced, or automatic when a step holds at least a
   // tenth of the coordinates (measured crossover on the 20,000-sample SVM:
