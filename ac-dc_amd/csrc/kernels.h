@@ -214,14 +214,19 @@ __global__ __launch_bounds__(kBlock) void k_step_scatter(StepArgs<T> a) {
 // copies the step's ncoord coefficients into LDS, adds every entry's
 // c_j a into an LDS accumulator of its f1 - f0 indices (LDS atomics: no
 // global memory operation inside the stream besides the entry loads, which
-// stay U per thread in flight), and ends with one coalesced read-modify-write
-// of res[f0, f1).
+// stay in flight a batch ahead), and ends with one coalesced
+// read-modify-write of res[f0, f1).
 template <typename T>
 struct TEntry {
   uint16_t lseg;   // residual index - f0 of the entry's block
   uint16_t coord;  // coordinate (column, or sample row)
   T val;
 };
+
+// threads per block of the streamed scatter: its LDS (the coefficients and
+// the accumulator) allows one block per CU, so the block is the CU's
+// sixteen waves
+constexpr int kTBlock = 1024;
 
 template <typename T>
 struct TScatterArgs {
@@ -234,49 +239,64 @@ struct TScatterArgs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_step_scatter_t(TScatterArgs<T> a) {
+__global__ __launch_bounds__(kTBlock) void k_step_scatter_t(TScatterArgs<T> a) {
   extern __shared__ __align__(16) unsigned char tl_raw[];
   T *cl = reinterpret_cast<T *>(tl_raw);  // coefficients, then the accumulator
   const int32_t e0 = a.eb[blockIdx.x], e1 = a.eb[blockIdx.x + 1];
   const int32_t f0 = a.fb[blockIdx.x], f1 = a.fb[blockIdx.x + 1];
   T *acc = cl + a.ncoord;
   constexpr int SU = 16;  // loads in flight per thread before their uses
-  for (int32_t i0 = threadIdx.x; i0 < a.ncoord; i0 += SU * kBlock) {
+  // a batch of SU entries per thread (lanes past e1 re-read the last entry
+  // and are masked where it is used)
+  auto load = [&](TEntry<T>(&e)[SU], int64_t q0) {
+#pragma unroll
+    for (int u = 0; u < SU; ++u) e[u] = a.ent[min(q0 + u * kTBlock, (int64_t)e1 - 1)];
+  };
+  // the first batch is in flight while the coefficients are staged, and each
+  // next batch while the current one is summed (memory counters drain in
+  // issue order: waiting for a batch does not wait for the one after it)
+  TEntry<T> cur[SU];
+  const int64_t q_first = (int64_t)e0 + threadIdx.x;
+  if (e0 < e1) load(cur, q_first);
+  for (int32_t i0 = threadIdx.x; i0 < a.ncoord; i0 += SU * kTBlock) {
     T v[SU];
 #pragma unroll
     for (int u = 0; u < SU; ++u) {
-      const int32_t i = i0 + u * kBlock;
+      const int32_t i = i0 + u * kTBlock;
       v[u] = i < a.ncoord ? a.cvec[i] : (T)0;
     }
 #pragma unroll
     for (int u = 0; u < SU; ++u) {
-      const int32_t i = i0 + u * kBlock;
+      const int32_t i = i0 + u * kTBlock;
       if (i < a.ncoord) cl[i] = v[u];
     }
   }
-  for (int32_t k = threadIdx.x; k < f1 - f0; k += kBlock) acc[k] = (T)0;
+  for (int32_t k = threadIdx.x; k < f1 - f0; k += kTBlock) acc[k] = (T)0;
   __syncthreads();
-  for (int32_t q0 = e0 + threadIdx.x; q0 < e1; q0 += SU * kBlock) {
-    TEntry<T> e[SU];
-#pragma unroll
-    for (int u = 0; u < SU; ++u) e[u] = a.ent[min((int64_t)q0 + u * kBlock, (int64_t)e1 - 1)];
+  for (int64_t q0 = q_first; q0 < e1; q0 += SU * kTBlock) {
+    TEntry<T> nxt[SU];
+    const bool more = q0 + SU * kTBlock < e1;
+    if (more) load(nxt, q0 + SU * kTBlock);
 #pragma unroll
     for (int u = 0; u < SU; ++u) {
-      const T c = cl[e[u].coord];
-      if ((int64_t)q0 + u * kBlock < e1 && c != (T)0) atomicAdd(&acc[e[u].lseg], c * e[u].val);
+      const T c = cl[cur[u].coord];
+      if (q0 + u * kTBlock < e1 && c != (T)0) atomicAdd(&acc[cur[u].lseg], c * cur[u].val);
     }
+    if (more)
+#pragma unroll
+      for (int u = 0; u < SU; ++u) cur[u] = nxt[u];
   }
   __syncthreads();
-  for (int32_t k0 = threadIdx.x; k0 < f1 - f0; k0 += SU * kBlock) {
+  for (int32_t k0 = threadIdx.x; k0 < f1 - f0; k0 += SU * kTBlock) {
     T r[SU];
 #pragma unroll
     for (int u = 0; u < SU; ++u) {
-      const int32_t k = k0 + u * kBlock;
+      const int32_t k = k0 + u * kTBlock;
       r[u] = k < f1 - f0 ? a.res[f0 + k] : (T)0;
     }
 #pragma unroll
     for (int u = 0; u < SU; ++u) {
-      const int32_t k = k0 + u * kBlock;
+      const int32_t k = k0 + u * kTBlock;
       if (k < f1 - f0 && acc[k] != (T)0) a.res[f0 + k] = r[u] + acc[k];
     }
   }

@@ -355,7 +355,7 @@ struct LaunchStep {
       t.cvec = a.cvec + (int64_t)(step & 1) * s->ncoord;
       t.ncoord = (int32_t)s->ncoord;
       t.res = a.res;
-      k_step_scatter_t<T><<<s->tgrid, kBlock, s->tlds, s->stream>>>(t);
+      k_step_scatter_t<T><<<s->tgrid, kTBlock, s->tlds, s->stream>>>(t);
     } else if (part & 2) {
       k_step_scatter<T, G><<<blocks, kBlock, 0, s->stream>>>(a);
     }
@@ -832,12 +832,12 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     s->wlanes = auto_eval_lanes(p->nnz, s->nres);
   }
   // streamed scatter: forced, or automatic when a step holds at least a
-  // tenth of the coordinates (measured crossover on the 20,000-sample SVM:
-  // tau ~1,800; one coalesced pass over every entry against one atomic per
-  // sampled entry)
+  // sixteenth of the coordinates (measured crossover on the 20,000-sample
+  // SVM: tau ~1,100; one coalesced pass over every entry against one atomic
+  // per sampled entry)
   s->streamed = !s->async && !s->persistent && set->tau >= 2 && stream_fits &&
                 (set->launch_mode == ACDC_LAUNCH_PER_STEP_STREAMED ||
-                 (set->launch_mode == ACDC_LAUNCH_AUTO && 10 * set->tau >= s->ncoord));
+                 (set->launch_mode == ACDC_LAUNCH_AUTO && 16 * set->tau >= s->ncoord));
   if (s->streamed) {
     int dev = 0, cus = 1;
     hipError_t te = hipGetDevice(&dev);

@@ -97,7 +97,7 @@ typedef struct acdc_settings {
 typedef enum acdc_launch_mode {
   ACDC_LAUNCH_AUTO = 0,      /* persistent cooperative grid when all tau slots
                                 fit on the device at once, else per step,
-                                with the streamed scatter when tau >= n/10
+                                with the streamed scatter when tau >= n/16
                                 and it applies (see below) */
   ACDC_LAUNCH_PER_STEP = 1,  /* one gather + one scatter launch per step */
   ACDC_LAUNCH_PERSISTENT = 2, /* persistent grid; fails if it does not fit */

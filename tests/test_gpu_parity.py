@@ -67,7 +67,7 @@ def test_setup_matches_oracle(acdc, orc, c1):
 
 
 def test_tau1_trajectory_fp64(acdc, orc, c1):
-    """north-star (b): tau = 1 FP64 matches the serial oracle within 1e-12."""
+    """tau = 1 FP64 matches the serial oracle within 1e-12, epoch by epoch."""
     p = c1.problem()
     s = acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=c1.lam, tau=1, seed=1)
     o = oracle_cd(orc, c1, 0, c1.lam)
@@ -246,12 +246,12 @@ def test_svm_matches_oracle(acdc, orc):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_streamed_scatter_svm(acdc, orc, prec):
-    """SVM minibatches with the streamed scatter (automatic at tau >= n/10,
+    """SVM minibatches with the streamed scatter (automatic at tau >= n/16,
     forced below) against the oracle's SDCA steps (FP64), and against the
     atomic per-step scatter on the same stream (FP32: the two differ only in
     the order of the FP32 additions into w)."""
     # rows of ~450 entries: automatic lanes give a block per row, which runs
-    # per step (no persistent grid), so tau = 512 >= n/10 streams by itself
+    # per step (no persistent grid), so tau = 512 >= n/16 streams by itself
     d = acdc.gen_svm(2000, 50_000, row_median=300.0, row_sigma=0.901, wstar_nnz=1000,
                      flip_frac=0.1, seed=5)
     p = d.problem()
