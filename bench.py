@@ -423,6 +423,23 @@ def svm_c3(acdc, args):
                                 "accuracy": r["accuracy"]})
             log(f"[bench] svm b={b} {name}: {20 * d.n / wall:.3e} upd/s, rel gap {r['rel_gap']:.2e}")
             s.close()
+    # time to relative gap 1e-6 from alpha = 0, gap checked every 5 epochs
+    # (checks included), FP32 + refresh against plain FP64, spectral beta
+    out["time_to_gap"] = []
+    for b in (512, 2048, 8192):
+        row = {"b": b}
+        for prec, name in ((acdc.PREC_FP32_REFRESH, "fp32"), (acdc.PREC_FP64, "fp64")):
+            s = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=b, seed=3,
+                            precision=prec, refresh_period=5, beta_mode=acdc.BETA_SPECTRAL)
+            tt, ep, rep = time_to_gap(s, d.n, args.gap_target, 5, 400)
+            row[name] = {"seconds": tt, "epochs": ep, "rel_gap": rep["rel_gap"],
+                         "reached": rep["rel_gap"] <= args.gap_target}
+            s.close()
+        if row["fp32"]["reached"] and row["fp64"]["reached"]:
+            row["speedup_fp32_vs_fp64"] = row["fp64"]["seconds"] / row["fp32"]["seconds"]
+        out["time_to_gap"].append(row)
+        log(f"[bench] svm b={b} time to gap: fp32 {row['fp32']['seconds']:.3f}s "
+            f"({row['fp32']['epochs']} epochs) vs fp64 {row['fp64']['seconds']:.3f}s")
     return out
 
 
