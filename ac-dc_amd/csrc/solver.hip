@@ -799,10 +799,11 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
                        "acdc_solver_create: tau * lanes_per_coord exceeds the co-resident grid "
                        "of the persistent kernel"));
   }
-  // SVM: the FP64 transposed copy of the refresh (w by feature, no atomics);
-  // every alpha_j is usually nonzero, so the scatter form would add all
-  // entries with FP64 atomics
-  if (loss == kSvm) {
+  // FP32 SVM: the FP64 transposed copy of the periodic refresh (w by
+  // feature, no atomics); every alpha_j is usually nonzero, so the scatter
+  // form would add all entries with FP64 atomics.  (The FP64 path refreshes
+  // only at creation and acdc_set_x, with the scatter form.)
+  if (loss == kSvm && s->fp32) {
     std::vector<int32_t> wp((size_t)s->nres + 1, 0);
     for (int32_t i : p->cidx) ++wp[(size_t)i + 1];
     for (int64_t i = 0; i < s->nres; ++i) wp[i + 1] += wp[i];
