@@ -231,6 +231,10 @@ struct acdc_solver {
   // event pairs around the update-kernel launches of one acdc_run_updates
   std::vector<hipEvent_t> step_ev;
   size_t step_ev_used = 0;
+  // pairs bracketing whole acdc_run_updates calls, read (with the step
+  // pairs) by settle_timing, so that a call returns once its work is queued
+  std::vector<hipEvent_t> run_ev;
+  size_t run_ev_used = 0;
   std::vector<void *> allocs;
 };
 
@@ -523,17 +527,53 @@ acdc_status timed_begin(acdc_solver *s) {
   HIP_TRY(hipEventRecord(s->ev0, s->stream));
   return ACDC_OK;
 }
-// records the next event of the solver's pool on its stream; pairs of marks
-// bracket the update-kernel launches of acdc_run_updates
-acdc_status step_mark(acdc_solver *s) {
-  if (s->step_ev_used == s->step_ev.size()) {
+// records the next event of an event pool on the solver's stream
+acdc_status pool_mark(acdc_solver *s, std::vector<hipEvent_t> &pool, size_t &used) {
+  if (used == pool.size()) {
     hipEvent_t e = nullptr;
     HIP_TRY(hipEventCreate(&e));
-    s->step_ev.push_back(e);
+    pool.push_back(e);
   }
-  HIP_TRY(hipEventRecord(s->step_ev[s->step_ev_used++], s->stream));
+  HIP_TRY(hipEventRecord(pool[used++], s->stream));
   return ACDC_OK;
 }
+// pairs of step marks bracket the update-kernel launches of acdc_run_updates
+acdc_status step_mark(acdc_solver *s) { return pool_mark(s, s->step_ev, s->step_ev_used); }
+acdc_status run_mark(acdc_solver *s) { return pool_mark(s, s->run_ev, s->run_ev_used); }
+
+// waits for the last pending run mark and adds the pending pairs to
+// kernel_ms (whole calls) and step_ms (update kernels); every mark is on the
+// solver's stream, so the last one completes after all the others
+acdc_status settle_timing(acdc_solver *s) {
+  if (s->run_ev_used == 0) return ACDC_OK;
+  HIP_TRY(hipEventSynchronize(s->run_ev[s->run_ev_used - 1]));
+  for (size_t e = 0; e + 1 < s->run_ev_used; e += 2) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, s->run_ev[e], s->run_ev[e + 1]));
+    s->kernel_ms += ms;
+  }
+  for (size_t e = 0; e + 1 < s->step_ev_used; e += 2) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, s->step_ev[e], s->step_ev[e + 1]));
+    s->step_ms += ms;
+  }
+  s->run_ev_used = 0;
+  s->step_ev_used = 0;
+  return ACDC_OK;
+}
+// a failed acdc_run_updates drops the marks it made, so pairs stay aligned
+struct PendingMarks {
+  acdc_solver *s;
+  size_t run0, step0;
+  bool keep = false;
+  ~PendingMarks() {
+    if (!keep) {
+      s->run_ev_used = run0;
+      s->step_ev_used = step0;
+    }
+  }
+};
+constexpr size_t kMaxPendingMarks = 512;
 
 acdc_status timed_end(acdc_solver *s, double *acc) {
   HIP_TRY(hipEventRecord(s->ev1, s->stream));
@@ -549,6 +589,7 @@ void destroy_solver(acdc_solver *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void *p : s->allocs) (void)hipFree(p);
   for (hipEvent_t e : s->step_ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->run_ev) (void)hipEventDestroy(e);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -955,7 +996,8 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
                               ? (uint64_t)s->set.refresh_period * (((uint64_t)s->ncoord + tau - 1) / tau)
                               : 0;
   acdc_status st;
-  if ((st = timed_begin(s)) != ACDC_OK) return st;
+  PendingMarks pending{s, s->run_ev_used, s->step_ev_used};
+  if ((st = run_mark(s)) != ACDC_OK) return st;
   uint64_t done = 0;
   while (done < nsteps) {
     uint64_t chunk = nsteps - done;
@@ -993,13 +1035,13 @@ extern "C" acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates) {
       if ((st = refresh(s)) != ACDC_OK) return st;
   }
   HIP_TRY(hipGetLastError());
-  if ((st = timed_end(s, &s->kernel_ms)) != ACDC_OK) return st;
-  for (size_t e = 0; e + 1 < s->step_ev_used; e += 2) {
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, s->step_ev[e], s->step_ev[e + 1]));
-    s->step_ms += ms;
-  }
-  s->step_ev_used = 0;
+  if ((st = run_mark(s)) != ACDC_OK) return st;
+  pending.keep = true;
+  // the call returns once its work is queued (the next call's launches
+  // follow on the stream without a gap); the persistent kernel's barrier
+  // status is read after it has finished
+  if (s->persistent || s->run_ev_used >= kMaxPendingMarks || s->step_ev_used >= kMaxPendingMarks)
+    if ((st = settle_timing(s)) != ACDC_OK) return st;
   if (s->persistent) {
     GridSync g{};
     HIP_TRY(hipMemcpy(&g, s->gsync, sizeof g, hipMemcpyDeviceToHost));
@@ -1019,6 +1061,7 @@ extern "C" acdc_status acdc_recompute_residuals(acdc_solver *s) {
 extern "C" acdc_status acdc_report_get(acdc_solver *s, acdc_report *out) {
   if (!s || !out) return fail(ACDC_ERR_INVALID, "acdc_report_get: null argument");
   acdc_status st;
+  if ((st = settle_timing(s)) != ACDC_OK) return st;
   if ((st = timed_begin(s)) != ACDC_OK) return st;
   if (s->fp32) {
     // stats come with the anchor; a refresh that no update has followed yet
@@ -1182,7 +1225,7 @@ extern "C" acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0, u
 extern "C" acdc_status acdc_synchronize(acdc_solver *s) {
   if (!s) return fail(ACDC_ERR_INVALID, "acdc_synchronize: null solver");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  return ACDC_OK;
+  return settle_timing(s);
 }
 
 extern "C" acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps, double *gather_ms,

@@ -181,7 +181,10 @@ acdc_status acdc_solver_destroy(acdc_solver *s);
 
 /* Runs ceil(n_updates / tau) steps of tau coordinates.
  * Replaces AbstractEngineExecutor::executeBulkOfIterations
- * (MulticoreEngineExecutor.h:83-89) with innerIterations = n_updates. */
+ * (MulticoreEngineExecutor.h:83-89) with innerIterations = n_updates.
+ * Returns once the work is queued on the solver's stream (persistent
+ * launches: once it has run), so consecutive calls run back to back; the
+ * report, the getters and acdc_synchronize wait for it. */
 acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates);
 
 /* r = Ax - b in FP64 (SVM: w = (1/(lambda n)) sum alpha_i y_i x_i).
