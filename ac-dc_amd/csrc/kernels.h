@@ -1010,26 +1010,28 @@ __global__ __launch_bounds__(kBlock) void k_eval_coord(Seg<T> A, int64_t nseg,
 
 // Folds nrows partial rows into out[kStats] with one block: thread t folds
 // rows t, t+256, ... in order, then a fixed pairwise tree.
-[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void k_reduce_rows(const double *__restrict__ partial,
-                                                        int64_t nrows, uint32_t maxmask,
-                                                        uint32_t minmask,
-                                                        double *__restrict__ out) {
-  __shared__ double sh[kBlock][kStats];
-  double v[kStats];
-  for (int s = 0; s < kStats; ++s) {
-    v[s] = (minmask & (1u << s)) ? INFINITY : ((maxmask & (1u << s)) ? -INFINITY : 0.0);
-  }
-  for (int64_t r = threadIdx.x; r < nrows; r += kBlock)
-    for (int s = 0; s < kStats; ++s) v[s] = fold(v[s], partial[r * kStats + s], s, maxmask, minmask);
-  for (int s = 0; s < kStats; ++s) sh[threadIdx.x][s] = v[s];
+// Folds nrows partial rows of kStats into out[0..kStats) in a fixed order:
+// thread t reads the flattened rows coalesced, from element t with a stride
+// of kReduceBlock (a multiple of kStats, so every element it reads is stat
+// t % kStats), then a tree over the threads of each stat.
+constexpr int kReduceBlock = 1024;
+static_assert(kReduceBlock % kStats == 0, "a thread folds one stat");
+[[maybe_unused]] static __global__ __launch_bounds__(kReduceBlock) void k_reduce_rows(
+    const double *__restrict__ partial, int64_t nrows, uint32_t maxmask, uint32_t minmask,
+    double *__restrict__ out) {
+  __shared__ double sh[kReduceBlock];
+  const int t = threadIdx.x, s = t % kStats;
+  double v = (minmask & (1u << s)) ? INFINITY : ((maxmask & (1u << s)) ? -INFINITY : 0.0);
+  const int64_t total = nrows * kStats;
+#pragma unroll 8
+  for (int64_t k = t; k < total; k += kReduceBlock) v = fold(v, partial[k], s, maxmask, minmask);
+  sh[t] = v;
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      for (int s = 0; s < kStats; ++s)
-        sh[threadIdx.x][s] = fold(sh[threadIdx.x][s], sh[threadIdx.x + w][s], s, maxmask, minmask);
+  for (int w = kReduceBlock / 2; w >= kStats; w >>= 1) {
+    if (t < w) sh[t] = fold(sh[t], sh[t + w], s, maxmask, minmask);
     __syncthreads();
   }
-  if (threadIdx.x < kStats) out[threadIdx.x] = sh[0][threadIdx.x];
+  if (t < kStats) out[t] = sh[t];
 }
 
 // --------------------------------------------------------------- sampling

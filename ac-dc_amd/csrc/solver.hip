@@ -482,7 +482,7 @@ struct LaunchEvalCoord {
     const int64_t blocks = std::min(kEvalBlocks, div_up(s->ncoord * G, kBlock));
     k_eval_coord<double, G, LOSS><<<(unsigned)blocks, kBlock, 0, s->stream>>>(
         seg64(s), s->ncoord, s->r64, s->x, s->rhs, gref, s->partial);
-    k_reduce_rows<<<1, kBlock, 0, s->stream>>>(s->partial, blocks, EvalMasks<LOSS>::max,
+    k_reduce_rows<<<1, kReduceBlock, 0, s->stream>>>(s->partial, blocks, EvalMasks<LOSS>::max,
                                                EvalMasks<LOSS>::min, s->stats);
   }
 };
@@ -1074,7 +1074,7 @@ extern "C" acdc_status acdc_report_get(acdc_solver *s, acdc_report *out) {
   const int64_t rblocks = std::min(kEvalBlocks, div_up(s->nres, kBlock));
   k_eval_res<<<(unsigned)rblocks, kBlock, 0, s->stream>>>(
       s->nres, s->r64, s->loss == kSvm ? nullptr : s->rhs, s->partial);
-  k_reduce_rows<<<1, kBlock, 0, s->stream>>>(s->partial, rblocks, 0u, 0u, s->stats + kStats);
+  k_reduce_rows<<<1, kReduceBlock, 0, s->stream>>>(s->partial, rblocks, 0u, 0u, s->stats + kStats);
   HIP_TRY(hipGetLastError());
   double h[2 * kStats];
   HIP_TRY(hipMemcpyAsync(h, s->stats, sizeof h, hipMemcpyDeviceToHost, s->stream));
