@@ -330,6 +330,8 @@ class Solver:
         self._h = h
 
     def run_updates(self, n_updates: int) -> None:
+        """Queues n_updates coordinate updates on the solver's stream and
+        returns; report(), x and synchronize() wait for them."""
         _check(lib.acdc_run_updates(self._h, int(n_updates)))
 
     def recompute_residuals(self) -> None:
