@@ -357,3 +357,28 @@ def test_warm_start_and_lambda_path(acdc, orc, c1):
     e = oracle_cd(orc, c1, 0, c1.lam).eval(s.x)
     assert s.report()["primal"] == pytest.approx(e["primal"], rel=1e-12)
     s.close()
+
+
+@pytest.mark.gpu
+def test_queued_runs_match_one_run(acdc, c1):
+    """acdc_run_updates returns once its work is queued and keeps each call's
+    device times as pending event pairs: 300 one-step calls (600 marks, past
+    the 512 after which a call folds them in) end on the same iterate as one
+    call of 300 steps, and the report counts every call's time.  tau = 1:
+    the serial path adds in one order (tau > 1 steps scatter with atomics,
+    whose order, and so the last bits, vary between runs)."""
+    p = c1.problem()
+    kw = dict(lam=c1.lam, tau=1, seed=9, precision=acdc.PREC_FP64)
+    many = acdc.Solver(p, acdc.LOSS_L1_SQUARE, **kw)
+    one = acdc.Solver(p, acdc.LOSS_L1_SQUARE, **kw)
+    for _ in range(300):
+        many.run_updates(1)
+    one.run_updates(300)
+    ra, rb = many.report(), one.report()
+    assert np.array_equal(many.x, one.x)
+    assert ra["primal"] == rb["primal"]
+    for r in (ra, rb):
+        assert r["updates"] == 300
+        assert 0 < r["step_ms"] <= r["kernel_ms"]
+    many.close()
+    one.close()
