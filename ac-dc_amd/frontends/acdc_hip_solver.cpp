@@ -14,11 +14,15 @@
 // and the HIP knobs (no reference counterpart):
 //   -s <seed>  -p <0 fp64 | 1 fp32+refresh>  -R <refresh epochs>  -L <lanes>
 //   -P <0 auto | 1 per-step launches | 2 persistent grid | 3 per step, streamed scatter>
-//   -M <0 sync | 1 async>
-//   -S <0|1>   spectral ESO beta (the power method of mSDCA.cpp:188-286)
+//   -M <0 sync | 1 async (default, the multicore executor's workers)>
+//   -B <-1 reference rule (default) | 0 omega | 2 none | 3 spectral>
+//   -S <0|1>   spectral ESO beta (the power method of mSDCA.cpp:188-286),
+//              the same as -B 3
 // In the reference, -I falls through into -M (option_console_parser.h:89-93);
-// here each flag sets only its own field.  omega and beta are computed as in
-// MultiCoreSolver.cpp:24-38, :55-62, :108-110.
+// here each flag sets only its own field.  omega and sigma are computed as in
+// MultiCoreSolver.cpp:24-38, :55-62, :108-110 and handed to the executor in
+// instance.sigma; under the reference rule the Lasso and NNLS ignore it
+// (MulticoreSquareLoss.h:27-40) and the SVM dual uses it.
 #include <getopt.h>
 
 #include <cstdio>
@@ -26,6 +30,7 @@
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "../host/acdc/HipEngineExecutor.h"
 #include "../host/acdc/Solver.h"
@@ -36,6 +41,7 @@ static void usage() {
                "                       [-I updates per bulk] [-c bulks] [-i 0|1] [-r log]\n"
                "                       [-s seed] [-p precision] [-R refresh epochs] [-L lanes]\n"
                "                       [-P launch mode] [-M step mode: 0 sync, 1 async]\n"
+               "                       [-B beta rule: -1 reference, 0 omega, 2 none, 3 spectral]\n"
                "                       [-S 1: spectral ESO beta]\n";
 }
 
@@ -45,7 +51,7 @@ int main(int argc, char *argv[]) {
   std::string matrixA, vectorB, logPath;
   double lambda = 0;
   int c;
-  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:s:p:R:L:P:M:S:h")) != -1) {
+  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:s:p:R:L:P:M:B:S:h")) != -1) {
     switch (c) {
       case 'A': matrixA = optarg; break;
       case 'b': vectorB = optarg; break;
@@ -68,7 +74,10 @@ int main(int argc, char *argv[]) {
       case 'L': settings.hipLanesPerCoord = std::atoi(optarg); break;
       case 'P': settings.hipLaunchMode = std::atoi(optarg); break;
       case 'M': settings.hipStepMode = std::atoi(optarg); break;
-      case 'S': settings.hipSpectralBeta = std::atoi(optarg) != 0; break;
+      case 'B': settings.hipBetaMode = std::atoi(optarg); break;
+      case 'S':
+        if (std::atoi(optarg) != 0) settings.hipBetaMode = ACDC_BETA_SPECTRAL;
+        break;
       default: usage(); return c == 'h' ? 0 : 2;
     }
   }
@@ -92,6 +101,19 @@ int main(int argc, char *argv[]) {
       InputOuputHelper::loadCSCData(matrixA, vectorB, lambda, instance);
     }
     std::cout << "Data loaded with size: " << instance.m << " x " << instance.n << std::endl;
+    // omega: the most entries of one row (CSC) or one feature (SVM rows),
+    // counted from 1 (MultiCoreSolver.cpp:24-38, :55-62)
+    {
+      const std::vector<int> &ix = settings.lossFunction == ACDC_LOSS_SVM_DUAL_HINGE
+                                       ? instance.A_csr_col_idx
+                                       : instance.A_csc_row_idx;
+      std::vector<int> counts((size_t)instance.m, 0);
+      double omega = 1;
+      for (int i : ix)
+        if (++counts[(size_t)i] > omega) omega = counts[(size_t)i];
+      std::cout << "Omega: " << omega << std::endl;
+      instance.sigma = 1 + (omega - 1) * (settings.totalThreads - 1) / (instance.n - 1.0);
+    }
     HipEngineExecutor<int, double> executor(instance, &settings);
     std::cout << "ESO beta: " << instance.sigma << std::endl;
     Solver<int, double> solver(executor);

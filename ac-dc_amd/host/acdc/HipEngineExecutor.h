@@ -56,6 +56,35 @@ std::vector<double> to_f64(const std::vector<D> &v) {
   return std::vector<double>(v.begin(), v.end());
 }
 
+// The ESO scaling a solver gets from settings.hipBetaMode (structures.h):
+// forcedSigma > 0 first (ClusterSolver.cpp:203-205); the reference rule --
+// beta-less Lasso/NNLS (MulticoreSquareLoss.h:27-40 ignores sigma), SVM with
+// the caller's sigma or the omega rule (MulticoreSVMDualLoss.h:48-61,
+// MultiCoreSolver.cpp:108-110); else the acdc_beta_mode asked for.
+inline void resolve_beta(int loss, const OptimizationSettings &set, double sigma,
+                         acdc_settings *s) {
+  s->forced_beta = 1.0;
+  if (set.forcedSigma > 0) {
+    s->beta_mode = ACDC_BETA_FORCED;
+    s->forced_beta = set.forcedSigma;
+  } else if (set.hipBetaMode == kHipBetaReference) {
+    if (loss != ACDC_LOSS_SVM_DUAL_HINGE) {
+      s->beta_mode = ACDC_BETA_NONE;
+    } else if (sigma > 0) {
+      s->beta_mode = ACDC_BETA_FORCED;
+      s->forced_beta = sigma;
+    } else {
+      s->beta_mode = ACDC_BETA_OMEGA;
+    }
+  } else if (set.hipBetaMode == ACDC_BETA_FORCED) {
+    if (!(sigma > 0)) throw std::runtime_error("hipBetaMode = ACDC_BETA_FORCED needs instance.sigma > 0");
+    s->beta_mode = ACDC_BETA_FORCED;
+    s->forced_beta = sigma;
+  } else {
+    s->beta_mode = set.hipBetaMode;
+  }
+}
+
 }  // namespace acdc_host
 
 template <typename L, typename D>
@@ -87,17 +116,7 @@ class HipEngineExecutor : public AbstractEngineExecutor<L, D> {
     s.lambda = (double)instance.lambda;
     s.tau = std::max(1, _settings->totalThreads);
     s.seed = _settings->hipSeed;
-    // an explicit beta wins: forcedSigma, else the caller's instance.sigma
-    // (MultiCoreSolver.cpp:108-110 fills it), else omega and tau
-    if (_settings->forcedSigma > 0) {
-      s.beta_mode = ACDC_BETA_FORCED;
-      s.forced_beta = _settings->forcedSigma;
-    } else if (instance.sigma > 0) {
-      s.beta_mode = ACDC_BETA_FORCED;
-      s.forced_beta = (double)instance.sigma;
-    } else {
-      s.beta_mode = _settings->hipSpectralBeta ? ACDC_BETA_SPECTRAL : ACDC_BETA_OMEGA;
-    }
+    resolve_beta(loss, *_settings, (double)instance.sigma, &s);
     s.precision = _settings->hipPrecision;
     s.step_mode = _settings->hipStepMode;
     s.lanes_per_coord = _settings->hipLanesPerCoord;
@@ -118,17 +137,32 @@ class HipEngineExecutor : public AbstractEngineExecutor<L, D> {
   }
 
   ~HipEngineExecutor() override {
+    // the caller's x ends current, as the reference's in-place x does
+    if (!this->settings->hipPullXEachBulk) {
+      try {
+        pull_x();
+      } catch (const std::exception &) {
+      }
+    }
     acdc_solver_destroy(solver_);
     acdc_problem_destroy(problem_);
   }
   HipEngineExecutor(const HipEngineExecutor &) = delete;
   HipEngineExecutor &operator=(const HipEngineExecutor &) = delete;
 
+  // returns when the bulk has run (the caller times it as pure computation,
+  // Solver.h:45-50); x is copied back when hipPullXEachBulk asks for it
   void executeBulkOfIterations() override {
     acdc_host::check(acdc_run_updates(solver_, (uint64_t)this->settings->innerIterations),
                      "acdc_run_updates");
-    pull_x();
+    if (this->settings->hipPullXEachBulk)
+      pull_x();
+    else
+      acdc_host::check(acdc_synchronize(solver_), "acdc_synchronize");
   }
+
+  // copies the device's x into instance.x
+  void syncX() { pull_x(); }
 
   void recomputeResiduals() override {
     acdc_host::check(acdc_recompute_residuals(solver_), "acdc_recompute_residuals");

@@ -13,6 +13,8 @@
 
 #include "../../../include/acdc_hip.h"
 
+constexpr int kHipBetaReference = -1;
+
 template <typename L, typename D>
 class ProblemData {
  public:
@@ -68,12 +70,27 @@ class OptimizationSettings {
   int hipRefreshPeriod = 5;                   // FP32 path: FP64 refresh every k epochs
   int hipLanesPerCoord = 0;                   // 0 = auto
   int hipLaunchMode = ACDC_LAUNCH_AUTO;
-  int hipStepMode = ACDC_STEP_SYNC;           // ACDC_STEP_ASYNC: the multicore executor's
-                                              // unsynchronised workers (tau = totalThreads)
-  bool hipSpectralBeta = false;               // beta from the largest eigenvalue of the
-                                              // normalised Gram matrix (mSDCA.cpp:188-286)
-                                              // unless forcedSigma / instance.sigma is set
+  int hipStepMode = ACDC_STEP_ASYNC;          // the multicore executor's unsynchronised
+                                              // workers, tau = totalThreads
+                                              // (MulticoreEngineExecutor.h:83-89);
+                                              // ACDC_STEP_SYNC: tau-nice synchronous steps
+                                              // (cdn_common.h:235-282)
+  int hipBetaMode = kHipBetaReference;        // step scaling, see below
+  bool hipPullXEachBulk = true;               // copy x into instance.x after every bulk (the
+                                              // reference executor updates the caller's x
+                                              // in place); false: only syncX() and the
+                                              // executor's destructor copy it, which saves
+                                              // an n-vector device-to-host copy per bulk
 };
+
+// hipBetaMode: kHipBetaReference (the default) applies the reference multicore
+// executor's rule -- the Lasso and NNLS take L_j = 1/||a_j||^2 and ignore
+// instance.sigma (MulticoreSquareLoss.h:27-40), the SVM dual takes
+// L_i = 1/(sigma ||x_i||^2) with the caller's instance.sigma
+// (MulticoreSVMDualLoss.h:48-61), or the omega rule of MultiCoreSolver.cpp:108-110
+// when the caller left sigma at 0.  Any acdc_beta_mode value applies that rule
+// to every loss instead (ACDC_BETA_FORCED takes instance.sigma).
+// settings.forcedSigma > 0 overrides both (ClusterSolver.cpp:203-205).
 
 class OptimizationStatistics {
  public:

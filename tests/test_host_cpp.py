@@ -43,9 +43,11 @@ def test_cli_lasso_matches_oracle(acdc, orc, tmp_path):
     write_csc(prefix, d)
     tau, bulks = 16, 6
     inner = 4 * d.n  # a multiple of tau
+    # synchronous tau-steps with the omega rule: replayed by the oracle
     r = subprocess.run([CLI, "-A", prefix, "-b", prefix + "_b.txt", "-l", repr(d.lam),
                         "-t", str(tau), "-f", "0", "-I", str(inner), "-c", str(bulks),
-                        "-i", "1", "-s", "21", "-r", str(tmp_path / "log.csv")],
+                        "-i", "1", "-s", "21", "-M", "0", "-B", "0",
+                        "-r", str(tmp_path / "log.csv")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     last = float(re.search(r"Last objective value: (\S+)", r.stdout).group(1))
@@ -71,3 +73,67 @@ def test_svm_executor_test_accuracy(acdc, tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.startswith("ok")
+
+
+@pytest.mark.gpu
+def test_cli_reference_rule_serial_matches_oracle(acdc, orc, tmp_path):
+    """Default flags: the reference multicore executor's rule -- asynchronous
+    workers, and the Lasso's L_j = 1/||a_j||^2 whatever sigma the front end
+    computed (MulticoreSquareLoss.h:27-40).  With one worker the run is the
+    serial method over worker 0's stream, which the oracle replays."""
+    d = acdc.gen_lasso(300, 1200, nnz_fixed=10, xstar_nnz=12, noise=0.01, lambda_frac=0.1,
+                       seed=19)
+    prefix = str(tmp_path / "lasso")
+    write_csc(prefix, d)
+    inner, bulks = 3 * d.n, 4
+    r = subprocess.run([CLI, "-A", prefix, "-b", prefix + "_b.txt", "-l", repr(d.lam),
+                        "-t", "1", "-f", "0", "-I", str(inner), "-c", str(bulks), "-i", "1",
+                        "-s", "23"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert float(re.search(r"ESO beta: (\S+)", r.stdout).group(1)) == 1.0
+    last = float(re.search(r"Last objective value: (\S+)", r.stdout).group(1))
+    o = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, d.lam, beta=1.0)
+    o.run_list(orc.async_coords(23, 0, 0, inner * bulks, d.n))
+    assert last == pytest.approx(o.eval()["primal"], rel=1e-12)
+
+
+@pytest.mark.gpu
+def test_cli_reference_rule_async_reaches_oracle_optimum(acdc, orc, tmp_path):
+    """tau = 16 asynchronous workers with beta = 1 (the reference multicore
+    Lasso at 16 threads): the last logged objective and duality gap reach the
+    optimum the serial oracle converges to."""
+    d = acdc.gen_lasso(300, 1200, nnz_fixed=10, xstar_nnz=12, noise=0.01, lambda_frac=0.1,
+                       seed=19)
+    prefix = str(tmp_path / "lasso")
+    write_csc(prefix, d)
+    r = subprocess.run([CLI, "-A", prefix, "-b", prefix + "_b.txt", "-l", repr(d.lam),
+                        "-t", "16", "-f", "0", "-I", str(50 * d.n), "-c", "6", "-i", "1",
+                        "-s", "5", "-r", str(tmp_path / "log.csv")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert float(re.search(r"ESO beta: (\S+)", r.stdout).group(1)) == 1.0
+    last = float(re.search(r"Last objective value: (\S+)", r.stdout).group(1))
+    o = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, d.lam, beta=1.0)
+    o.run(seed=5, tau=1, step0=0, nsteps=400 * d.n)
+    opt = o.eval()
+    assert opt["rel_gap"] < 1e-12
+    assert last == pytest.approx(opt["primal"], rel=1e-9)
+    # the Lasso row's last column is the duality gap
+    gap = float(open(tmp_path / "log.csv").read().strip().splitlines()[-1].split(",")[-1])
+    assert 0 <= gap <= 1e-9 * opt["primal"]
+
+
+@pytest.mark.gpu
+def test_executor_beta_rules(acdc, tmp_path):
+    """HipEngineExecutor's step scaling: the reference rule (Lasso/NNLS
+    L_j = 1/||a_j||^2 at any thread count; SVM 1/(sigma ||x_i||^2) with the
+    caller's sigma), an explicit omega rule, and forcedSigma."""
+    exe = tmp_path / "test_executor_beta"
+    lib = os.path.join(ROOT, "ac-dc_amd", "lib")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror",
+                    "-I", os.path.join(ROOT, "ac-dc_amd", "host"),
+                    os.path.join(ROOT, "tests", "cpp", "test_executor_beta.cpp"),
+                    "-L", lib, "-lacdc_hip", f"-Wl,-rpath,{lib}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip().endswith("ok")
