@@ -15,8 +15,11 @@ The objects mirror the reference's solver interface:
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -27,6 +30,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libacdc_hip.so")
 LOSS_L1_SQUARE = 0
 LOSS_SVM_DUAL_HINGE = 2
 LOSS_NNLS = 10
+BATCH_NONFINITE = 1
 PREC_FP64 = 0
 PREC_FP32_REFRESH = 1
 BETA_OMEGA = 0
@@ -39,6 +43,8 @@ LAUNCH_AUTO = 0
 LAUNCH_PER_STEP = 1
 LAUNCH_PERSISTENT = 2
 LAUNCH_PER_STEP_STREAMED = 3
+LAUNCH_ASYNC_DIRECT = 4
+LAUNCH_ASYNC_STAGED = 5
 
 NNZ_FIXED = 0
 NNZ_ZIPF_TRUNC = 1
@@ -101,13 +107,15 @@ class BatchSettings(ctypes.Structure):
                 ("max_epochs", ctypes.c_int32), ("gap_tol", ctypes.c_double),
                 ("use_graph", ctypes.c_int32), ("step_mode", ctypes.c_int32),
                 ("tau", ctypes.c_int32), ("queue_chunk", ctypes.c_int32),
-                ("loss", ctypes.c_int32), ("beta_mode", ctypes.c_int32)]
+                ("loss", ctypes.c_int32), ("beta_mode", ctypes.c_int32),
+                ("screening", ctypes.c_int32)]
 
 
 class _BatchResult(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ("lambda_", "primal", "dual", "rel_gap", "nnz_x")] + \
                [("epochs", ctypes.c_int32), ("converged", ctypes.c_int32),
-                ("updates", ctypes.c_uint64), ("device_us", ctypes.c_double)]
+                ("updates", ctypes.c_uint64), ("device_us", ctypes.c_double),
+                ("ws_size", ctypes.c_double), ("status", ctypes.c_int32)]
 
 
 def _load() -> ctypes.CDLL:
@@ -171,6 +179,26 @@ for _name, (_args, _res) in _SIGS.items():
     _fn = getattr(lib, _name)
     _fn.argtypes = _args
     _fn.restype = _res
+
+
+# Every live handle, so that they are all released by an atexit hook -- which
+# runs before the interpreter tears modules down and before the C runtime's
+# exit handlers (a profiler's finalisation among them) -- instead of by
+# __del__ during interpreter shutdown.  Solvers and batches go before the
+# problems they were built from.
+_live_solvers: "weakref.WeakSet" = weakref.WeakSet()
+_live_problems: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def close_all() -> None:
+    """Destroys every solver, batch and problem handle still open."""
+    for obj in list(_live_solvers):
+        obj.close()
+    for obj in list(_live_problems):
+        obj.close()
+
+
+atexit.register(close_all)
 
 
 def _check(status: int) -> None:
@@ -270,6 +298,7 @@ class Problem:
     def __init__(self, handle: _P, kind: int, m: int, n: int, nnz: int):
         self._h = handle
         self.kind, self.m, self.n, self.nnz = kind, m, n, nnz
+        _live_problems.add(self)
 
     @classmethod
     def from_csc(cls, m, n, col_ptr, row_idx, vals, b) -> "Problem":
@@ -307,7 +336,8 @@ class Problem:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not sys.is_finalizing():  # at shutdown the atexit hook has run
+            self.close()
 
 
 class Solver:
@@ -328,6 +358,7 @@ class Solver:
         h = _P()
         _check(lib.acdc_solver_create(problem._h, loss, ctypes.byref(st), ctypes.byref(h)))
         self._h = h
+        _live_solvers.add(self)
 
     def run_updates(self, n_updates: int) -> None:
         """Queues n_updates coordinate updates on the solver's stream and
@@ -394,7 +425,8 @@ class Solver:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not sys.is_finalizing():
+            self.close()
 
 
 class Batch:
@@ -406,7 +438,7 @@ class Batch:
     def __init__(self, problems, lambdas, *, seed=1, precision=PREC_FP32_REFRESH,
                  lanes_per_coord=0, refresh_period=5, max_epochs=200, gap_tol=1e-6,
                  use_graph=True, step_mode=STEP_SYNC, tau=0, queue_chunk=0,
-                 loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA):
+                 loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA, screening=False):
         lam = _f64(lambdas)
         if lam.ndim != 2 or lam.shape[0] != len(problems):
             raise ValueError("lambdas must be (instances, path_len)")
@@ -415,11 +447,13 @@ class Batch:
         self.n = [p.n for p in problems]
         arr = (_P * self.count)(*[p._h for p in problems])
         st = BatchSettings(seed, precision, lanes_per_coord, refresh_period, max_epochs, gap_tol,
-                           int(bool(use_graph)), step_mode, tau, queue_chunk, loss, beta_mode)
+                           int(bool(use_graph)), step_mode, tau, queue_chunk, loss, beta_mode,
+                           int(bool(screening)))
         h = _P()
         _check(lib.acdc_batch_create(arr, self.count, _ptr(lam, ctypes.c_double), self.path_len,
                                      ctypes.byref(st), ctypes.byref(h)))
         self._h = h
+        _live_solvers.add(self)
 
     def run(self) -> float:
         """Solves the whole batch from x = 0; returns the device time in ms."""
@@ -432,7 +466,8 @@ class Batch:
         _check(lib.acdc_batch_results(self._h, out, self.count * self.path_len))
         dt = np.dtype([("lambda", "f8"), ("primal", "f8"), ("dual", "f8"), ("rel_gap", "f8"),
                        ("nnz_x", "f8"), ("epochs", "i4"), ("converged", "i4"), ("updates", "u8"),
-                       ("device_us", "f8")])
+                       ("device_us", "f8"), ("ws_size", "f8"), ("status", "i4"),
+                       ("_pad", "i4")])
         return np.frombuffer(bytes(out), dtype=dt).reshape(self.count, self.path_len).copy()
 
     def x(self, i: int) -> np.ndarray:
@@ -452,16 +487,18 @@ class Batch:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not sys.is_finalizing():
+            self.close()
 
 
 def batch_tune(problems, *, seed=1, precision=PREC_FP32_REFRESH, step_mode=STEP_ASYNC,
                refresh_period=5, max_epochs=200, gap_tol=1e-6, use_graph=True,
-               loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA) -> dict:
+               loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA, screening=False) -> dict:
     """acdc_batch_tune: lanes per coordinate, tau and queue chunking for a batch."""
     arr = (_P * len(problems))(*[p._h for p in problems])
     base = BatchSettings(seed, precision, 0, refresh_period, max_epochs, gap_tol,
-                         int(bool(use_graph)), step_mode, 0, 0, loss, beta_mode)
+                         int(bool(use_graph)), step_mode, 0, 0, loss, beta_mode,
+                         int(bool(screening)))
     out = BatchSettings()
     score = ctypes.c_double()
     _check(lib.acdc_batch_tune(arr, len(problems), ctypes.byref(base), ctypes.byref(out),

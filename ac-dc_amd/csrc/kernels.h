@@ -18,6 +18,8 @@
 // shrinks with the step sizes, so the iteration converges to the solution of
 // the FP64 problem (iterative refinement), not of the FP32-rounded one.
 #pragma once
+#include <type_traits>
+
 #include "device_common.h"
 
 namespace acdc {
@@ -544,7 +546,8 @@ __device__ __forceinline__ double rec_gref(const AsyncRec<double> &) { return 0.
 template <typename T>
 struct AsyncArgs {
   AsyncRec<T> *rec;
-  const Entry<T> *ent;
+  const Entry<T> *ent;  // nnz entries and two of zero padding
+  int64_t nnz;
   T *res;  // FP64: residual; FP32: correction dr
   StepScalars sc;
   uint64_t seed;
@@ -672,6 +675,268 @@ __global__ __launch_bounds__(kAsyncBlock) void k_async(AsyncArgs<T> a, uint64_t 
   for (uint64_t k = 0; k < count; k += 2) {
     update(k, s0, s1, ahead);
     if (k + 1 < count) update(k + 1, s1, s0, ahead);
+  }
+}
+
+// ------------------------------------------------ staged asynchronous workers
+// The same workers as k_async, one per 64-lane wave, with the data of each
+// update staged in LDS by a loader wave so that a worker waits on nothing but
+// its residual gathers.
+//
+// Why: a wave's vector-memory counter retires in issue order, so in k_async
+// the gathers of update k wait for the record and entry prefetches of the
+// next updates issued before them -- an HBM round trip per update (~2 us
+// under load, against ~0.12 us for an L2-resident gather).  Here a block is
+// kStagedWorkers worker waves plus one loader wave that walks the workers'
+// coordinate streams ahead of them and copies, with global_load_lds_dwordx4
+// (straight into LDS, no registers), each coming update's 32-byte record
+// kStagedLeadRec phases before its use and, kStagedLead phases before its
+// use, its entries (the bounds come from the record copied three phases
+// earlier).  One s_barrier per phase keeps the rings in step: at phase p
+// worker w handles its update p - kStagedLeadRec.
+//
+// The loader issues the same kStagedLoadsPerPhase copies every phase
+// (lanes, updates and workers out of range re-load valid addresses), so
+// "s_waitcnt vmcnt(2 * kStagedLoadsPerPhase)" at the end of a phase retires
+// exactly the copies issued two phases before.  The copies are written in
+// inline assembly: the compiler then does not see LDS writes in flight and
+// does not wait for all of them before the loader's own LDS reads of the
+// bounds (which the explicit waits already order).  The loader issues no
+// other vector-memory instruction.
+//
+// A worker reads its record and entries from LDS, reads x_j and gathers the
+// residual with agent-scope loads, and applies the update as k_async does
+// (hardware atomics; compare-and-swap for the constrained losses).
+//
+// LDS: kStagedDepth entry slots -- per worker kStagedSlotBytes (FP32: 256
+// entries from the even entry at or below the segment start; FP64: 128) --
+// and kStagedRecDepth record slots of 1 KiB (8 records in the first 256
+// bytes; the other lanes of the copy write duplicates behind them).
+// Segments longer than a slot read the rest from global memory (TAIL).
+constexpr int kStagedWorkers = 8;
+constexpr int kStagedBlock = 64 * (kStagedWorkers + 1);
+constexpr int kStagedDepth = 4;                  // entry slots
+constexpr int kStagedLead = kStagedDepth - 1;    // phases from an entry copy to its use
+constexpr int kStagedLeadRec = 2 * kStagedLead;  // phases from a record copy to its use
+constexpr int kStagedRecDepth = 8;               // record slots (> kStagedLeadRec)
+constexpr int kStagedSlotBytes = 2048;
+constexpr int kStagedRecBytes = 1024;
+constexpr int kStagedLds =
+    kStagedDepth * kStagedWorkers * kStagedSlotBytes + kStagedRecDepth * kStagedRecBytes;
+// copies per phase: one record copy (16 lanes hold the 8 records), two entry
+// copies per worker
+constexpr int kStagedLoadsPerPhase = 1 + 2 * kStagedWorkers;
+static_assert(3 * kStagedLoadsPerPhase <= 63, "the vector-memory counter holds 63 loads");
+static_assert(kStagedRecDepth > kStagedLeadRec, "a record slot outlives its use");
+// entries of a segment a slot holds whatever its alignment (FP32: the copy
+// starts at the even entry at or below the segment's first)
+template <typename T>
+constexpr int kStagedCached = kStagedSlotBytes / (int)sizeof(Entry<T>) - (sizeof(T) == 4 ? 1 : 0);
+
+// 16 bytes per lane from global memory to LDS byte address lds + 16 * lane
+// (lds wave-uniform).  The instruction has no result register.
+__device__ __forceinline__ void copy16_to_lds(const void *src, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds)
+               : "memory", "m0");
+}
+
+// Sum over the whole wave with VALU lane moves only (DPP inside rows of 16,
+// then row and half swaps), so no LDS instruction waits behind outstanding
+// loads.  Every lane adds the same two values in the same order at each
+// step, so all lanes receive the bitwise-identical total.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  v += dpp_move<kDppXor1>(v);
+  v += dpp_move<kDppXor2>(v);
+  v += dpp_move<kDppHalfMirror>(v);
+  v += dpp_move<kDppMirror>(v);
+  auto swap_sum = [](T x, auto swap) -> T {
+    if constexpr (sizeof(T) == 4) {
+      const auto r = swap(__float_as_uint(x), __float_as_uint(x));
+      return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    } else {
+      const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+      const auto lo = swap((unsigned)b, (unsigned)b);
+      const auto hi = swap((unsigned)(b >> 32), (unsigned)(b >> 32));
+      const double d0 = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+      const double d1 = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+      return d0 + d1;
+    }
+  };
+  // rows 0,1 <-> rows 2,3 pairs: lanes of rows 0 and 1 get r0 + r1, of rows 2 and 3 r2 + r3
+  v = swap_sum(v, [](unsigned a, unsigned b) { return __builtin_amdgcn_permlane16_swap(a, b, false, false); });
+  // halves: every lane gets (r0 + r1) + (r2 + r3)
+  v = swap_sum(v, [](unsigned a, unsigned b) { return __builtin_amdgcn_permlane32_swap(a, b, false, false); });
+  return v;
+}
+
+// The update of one worker wave once its partial dot is reduced: the
+// proximal step on x_j (hardware atomic add for the Lasso, compare-and-swap
+// against the x_j it replaces for the constrained losses, as in k_async) and
+// the coefficient of the residual scatter.
+template <typename T, int LOSS>
+__device__ __forceinline__ double async_apply(const AsyncArgs<T> &a, uint32_t j, double g,
+                                              double xj, double Lj, double yj, int lane) {
+  double c;
+  if constexpr (LOSS == kLasso) {
+    const double d = prox_delta<LOSS>(g, xj, Lj, yj, a.sc, &c);
+    if (lane == 0 && d != 0.0) unsafeAtomicAdd(&a.rec[j].x, d);
+  } else {
+    double d = 0.0;
+    if (lane == 0) {
+      unsigned long long *px = reinterpret_cast<unsigned long long *>(&a.rec[j].x);
+      double old = xj;
+      for (;;) {
+        double cc;
+        const double dd = prox_delta<LOSS>(g, old, Lj, yj, a.sc, &cc);
+        const double nw = old + dd;
+        if (nw == old) {
+          d = dd;
+          break;
+        }
+        const unsigned long long prev =
+            atomicCAS(px, __double_as_longlong(old), __double_as_longlong(nw));
+        if (prev == (unsigned long long)__double_as_longlong(old)) {
+          d = dd;
+          break;
+        }
+        old = __longlong_as_double((long long)prev);
+      }
+    }
+    d = __shfl(d, 0, 64);
+    c = LOSS == kSvm ? d * yj * a.sc.inv_lambda_n : d;
+  }
+  return c;
+}
+
+template <typename T, int LOSS, bool TAIL>
+__global__ __launch_bounds__(kStagedBlock) void k_async_staged(AsyncArgs<T> a, uint64_t k0,
+                                                               uint64_t count) {
+  __shared__ __attribute__((aligned(16))) unsigned char ring[kStagedLds];
+  constexpr int kSlotEntries = kStagedSlotBytes / (int)sizeof(Entry<T>);
+  constexpr int R = kSlotEntries / 64;  // slot entries per lane
+  const int64_t nphase = (int64_t)count + kStagedLeadRec;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned char *const rec_ring = ring + kStagedDepth * kStagedWorkers * kStagedSlotBytes;
+  if (count == 0) return;  // uniform: every wave leaves before any barrier
+
+  if (wave < kStagedWorkers) {
+    // ---- worker wave
+    const int64_t w = (int64_t)blockIdx.x * kStagedWorkers + wave;
+    const bool active = w < a.tau;
+    const bool serial = a.tau == 1;
+    AsyncCoordCache<64> coords;
+    for (int64_t ph = 0; ph < nphase; ++ph) {
+      const int64_t u = ph - kStagedLeadRec;
+      if (active && u >= 0) {
+        const uint32_t j = coords.get(a.seed, (uint32_t)w, k0 + (uint64_t)u, a.n);
+        const AsyncRec<T> r = *(const AsyncRec<T> *)(rec_ring +
+                                                     (int)(u % kStagedRecDepth) * kStagedRecBytes +
+                                                     wave * 32);
+        const int off = sizeof(T) == 4 ? (r.p0 & 1) : 0;
+        const Entry<T> *eb =
+            (const Entry<T> *)(ring + ((int)(u % kStagedDepth) * kStagedWorkers + wave) *
+                                          kStagedSlotBytes) + off;
+        const int32_t cached = min(r.len, kSlotEntries - off);
+        int32_t ci[R];
+        T cv[R], v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {  // all slot reads at once; entries past the segment zeroed
+          const Entry<T> e = eb[lane + 64 * k];
+          const bool in = lane + 64 * k < cached;
+          ci[k] = in ? e.row : 0;
+          cv[k] = in ? e.val : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = lane + 64 * k < cached ? ld_coherent(&a.res[ci[k]]) : (T)0;
+        const double xj = ld_coherent(&a.rec[j].x);
+        T acc = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) acc += cv[k] * v[k];
+        if (TAIL)  // entries beyond the slot, four chunks per round trip
+          for (int32_t q0 = cached + lane; q0 < r.len; q0 += 256) {
+            Entry<T> e[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) e[k] = a.ent[r.p0 + min(q0 + 64 * k, r.len - 1)];
+            T g4[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) g4[k] = ld_coherent(&a.res[e[k].row]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+              if (q0 + 64 * k < r.len) acc += e[k].val * g4[k];
+          }
+        acc = wave_sum(acc);
+        const double c = async_apply<T, LOSS>(a, j, rec_gref(r) + (double)acc, xj, rec_L(r),
+                                              (double)r.y, lane);
+        if (c != 0.0) {
+          const T ct = (T)c;
+#pragma unroll
+          for (int k = 0; k < R; ++k)
+            if (lane + 64 * k < cached) unsafeAtomicAdd(&a.res[ci[k]], ct * cv[k]);
+          if (TAIL)
+            for (int32_t q = cached + lane; q < r.len; q += 64) {
+              const Entry<T> e = a.ent[r.p0 + q];
+              unsafeAtomicAdd(&a.res[e.row], ct * e.val);
+            }
+        }
+        if (serial) wait_memory();  // one worker: the serial method
+      }
+      __builtin_amdgcn_s_barrier();
+    }
+  } else {
+    // ---- loader wave.  Phase ph: copy the record of update ph
+    // (kStagedLeadRec phases ahead of the workers) into record slot
+    // ph mod kStagedRecDepth, then the entries of update ph - kStagedLead,
+    // whose record was copied kStagedLead phases ago, into entry slot
+    // (ph - kStagedLead) mod kStagedDepth.
+    const uint32_t ring0 = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) unsigned char *)ring;
+    const uint32_t rec0 = ring0 + kStagedDepth * kStagedWorkers * kStagedSlotBytes;
+    const int rw = (lane & 15) >> 1;  // record copy: worker of this lane (two lanes per record)
+    const int64_t w = (int64_t)blockIdx.x * kStagedWorkers + rw;
+    const uint32_t wid = w < a.tau ? (uint32_t)w : 0u;
+    const int bw = lane & (kStagedWorkers - 1);  // bounds read: worker of this lane
+    const int64_t last_u = (int64_t)count - 1;
+    for (int64_t ph = 0; ph < nphase; ++ph) {
+      {
+        const uint64_t u = (uint64_t)min(ph, last_u);
+        const uint32_t j = async_coord(a.seed, wid, k0 + u, a.n);
+        copy16_to_lds(reinterpret_cast<const char *>(&a.rec[j]) + 16 * (lane & 1),
+                      rec0 + (uint32_t)(ph % kStagedRecDepth) * kStagedRecBytes);
+      }
+      const int64_t u2 = ph - kStagedLead;
+      int2 b = make_int2(0, 0);  // (offset, length) of worker bw's segment
+      if (u2 >= 0)
+        b = *(const int2 *)(rec_ring + (int)(min(u2, last_u) % kStagedRecDepth) * kStagedRecBytes +
+                            bw * 32 + 24);
+      const uint32_t slot0 =
+          ring0 + (uint32_t)((((u2 % kStagedDepth) + kStagedDepth) % kStagedDepth) * kStagedWorkers) *
+                      kStagedSlotBytes;
+#pragma unroll
+      for (int i = 0; i < kStagedWorkers; ++i) {
+        // bounds kept inside the entry array whatever LDS holds
+        const int32_t p0 = min(max(__builtin_amdgcn_readlane(b.x, i), 0), (int32_t)a.nnz);
+        const int32_t len = min(max(__builtin_amdgcn_readlane(b.y, i), 0), (int32_t)a.nnz - p0);
+        const uint32_t dst = slot0 + (uint32_t)i * kStagedSlotBytes;
+        if constexpr (sizeof(T) == 4) {
+          // entry pairs from the even entry at or below p0; lanes past the
+          // segment re-load its last pair
+          const int32_t base = p0 & ~1;
+          const int32_t last = max(0, (p0 + len - 1) & ~1) - base;
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            copy16_to_lds(a.ent + base + min(2 * (64 * c + lane), last), dst + 1024 * c);
+        } else {
+          const int32_t last = max(0, len - 1);
+#pragma unroll
+          for (int c = 0; c < 2; ++c) copy16_to_lds(a.ent + p0 + min(64 * c + lane, last), dst + 1024 * c);
+        }
+      }
+      // retire the copies of two phases ago: the entries the workers use
+      // next phase and the records whose bounds are read next phase
+      asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * kStagedLoadsPerPhase) : "memory");
+      __builtin_amdgcn_s_barrier();
+    }
+    wait_memory();  // no copy into LDS may outlive the block
   }
 }
 

@@ -192,6 +192,7 @@ struct acdc_solver {
   bool fp32 = false;
   bool persistent = false;
   bool async = false;  // ACDC_STEP_ASYNC
+  bool staged = false;  // asynchronous workers fed through LDS (k_async_staged)
   // per-step launches whose scatter streams the transposed matrix
   // (k_step_scatter_t): its entries, the two coefficient halves, the grid
   bool streamed = false;
@@ -373,11 +374,22 @@ struct LaunchAsync {
     AsyncArgs<T> a;
     a.rec = static_cast<AsyncRec<T> *>(s->rec);
     a.ent = static_cast<const Entry<T> *>(s->ent);
+    a.nnz = s->nnz;
     a.res = static_cast<T *>(s->res);
     a.sc = scalars(s);
     a.seed = s->set.seed;
     a.tau = s->set.tau;
     a.n = (uint32_t)s->ncoord;
+    if constexpr (G == 64) {
+      if (s->staged) {
+        const unsigned sb = (unsigned)div_up(s->set.tau, kStagedWorkers);
+        if (s->maxlen <= kStagedCached<T>)
+          k_async_staged<T, LOSS, false><<<sb, kStagedBlock, 0, s->stream>>>(a, k0, count);
+        else
+          k_async_staged<T, LOSS, true><<<sb, kStagedBlock, 0, s->stream>>>(a, k0, count);
+        return;
+      }
+    }
     const unsigned blocks = (unsigned)div_up(s->set.tau * G, kAsyncBlock);
     if (s->maxlen <= kAsyncChunks * G)
       k_async<T, G, LOSS, kAsyncChunks, false><<<blocks, kAsyncBlock, 0, s->stream>>>(a, k0, count);
@@ -544,19 +556,24 @@ acdc_status run_mark(acdc_solver *s) { return pool_mark(s, s->run_ev, s->run_ev_
 // waits for the last pending run mark and adds the pending pairs to
 // kernel_ms (whole calls) and step_ms (update kernels); every mark is on the
 // solver's stream, so the last one completes after all the others
+// (sums are committed, and the marks released, only when every pair reads
+// back, so that a failed call can be retried without counting pairs twice)
 acdc_status settle_timing(acdc_solver *s) {
   if (s->run_ev_used == 0) return ACDC_OK;
   HIP_TRY(hipEventSynchronize(s->run_ev[s->run_ev_used - 1]));
+  double run_ms = 0, steps_ms = 0;
   for (size_t e = 0; e + 1 < s->run_ev_used; e += 2) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, s->run_ev[e], s->run_ev[e + 1]));
-    s->kernel_ms += ms;
+    run_ms += ms;
   }
   for (size_t e = 0; e + 1 < s->step_ev_used; e += 2) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, s->step_ev[e], s->step_ev[e + 1]));
-    s->step_ms += ms;
+    steps_ms += ms;
   }
+  s->kernel_ms += run_ms;
+  s->step_ms += steps_ms;
   s->run_ev_used = 0;
   s->step_ev_used = 0;
   return ACDC_OK;
@@ -694,8 +711,15 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown precision");
   if (set->step_mode != ACDC_STEP_SYNC && set->step_mode != ACDC_STEP_ASYNC)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown step mode");
-  if (set->launch_mode < ACDC_LAUNCH_AUTO || set->launch_mode > ACDC_LAUNCH_PER_STEP_STREAMED)
+  if (set->launch_mode < ACDC_LAUNCH_AUTO || set->launch_mode > ACDC_LAUNCH_ASYNC_STAGED)
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: unknown launch mode");
+  if (set->launch_mode >= ACDC_LAUNCH_ASYNC_DIRECT && set->step_mode != ACDC_STEP_ASYNC)
+    return fail(ACDC_ERR_INVALID,
+                "acdc_solver_create: launch modes 4 and 5 are for asynchronous workers");
+  if (set->launch_mode == ACDC_LAUNCH_ASYNC_STAGED && set->lanes_per_coord != 0 &&
+      set->lanes_per_coord != 64)
+    return fail(ACDC_ERR_INVALID,
+                "acdc_solver_create: staged asynchronous workers take 64 lanes per coordinate");
   const size_t coef_bytes = (size_t)p->ncoord * (set->precision == ACDC_PREC_FP32_REFRESH ? 4 : 8);
   const bool stream_fits = coef_bytes <= kStreamCoefBytes && p->ncoord <= 65535;
   if (set->launch_mode == ACDC_LAUNCH_PER_STEP_STREAMED &&
@@ -746,7 +770,9 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
   s->nnz = p->nnz;
   s->beta = beta;
   s->fp32 = set->precision == ACDC_PREC_FP32_REFRESH;
-  s->lanes = set->lanes_per_coord ? set->lanes_per_coord
+  s->staged = set->step_mode == ACDC_STEP_ASYNC && set->launch_mode == ACDC_LAUNCH_ASYNC_STAGED;
+  s->lanes = s->staged ? 64
+             : set->lanes_per_coord ? set->lanes_per_coord
                                    : (set->step_mode == ACDC_STEP_SYNC &&
                                               set->launch_mode != ACDC_LAUNCH_PERSISTENT
                                           ? auto_step_lanes(p->nnz, p->ncoord)
@@ -791,7 +817,11 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     const size_t rsz = s->fp32 ? sizeof(AsyncRec<float>) : sizeof(AsyncRec<double>);
     const size_t esz = s->fp32 ? sizeof(Entry<float>) : sizeof(Entry<double>);
     if ((st = dalloc(s, &s->rec, (size_t)s->ncoord * rsz)) != ACDC_OK) return bail(st);
-    if ((st = dalloc(s, &s->ent, (size_t)p->nnz * esz)) != ACDC_OK) return bail(st);
+    // two entries of zero padding: the staged loader copies whole 16-byte
+    // pairs and may touch the pair after the last segment
+    if ((st = dalloc(s, &s->ent, ((size_t)p->nnz + 2) * esz)) != ACDC_OK) return bail(st);
+    if (hipMemset(static_cast<char *>(s->ent) + (size_t)p->nnz * esz, 0, 2 * esz) != hipSuccess)
+      return bail(fail(ACDC_ERR_HIP, "acdc_solver_create: entry padding"));
   }
 
   hipError_t e = hipMemcpy(s->ptr, p->cptr.data(), p->cptr.size() * 4, hipMemcpyHostToDevice);
@@ -1177,6 +1207,11 @@ extern "C" acdc_status acdc_set_lambda(acdc_solver *s, double lambda) {
     const double f = s->set.lambda / lambda;
     const unsigned rb = (unsigned)div_up(s->nres, kBlock);
     if (s->fp32) {
+      // asynchronous workers keep the live alpha in the records: bring it to
+      // the plain array first, since the records are rebuilt from that array
+      // (with the rescaled anchor) before the next updates
+      acdc_status st = ensure_x(s);
+      if (st != ACDC_OK) return st;
       k_scale<float><<<rb, kBlock, 0, s->stream>>>(static_cast<float *>(s->res), s->nres, f);
       k_scale<double><<<rb, kBlock, 0, s->stream>>>(s->r64, s->nres, f);
       k_scale<double><<<(unsigned)div_up(s->ncoord, kBlock), kBlock, 0, s->stream>>>(

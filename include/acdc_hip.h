@@ -90,10 +90,11 @@ typedef struct acdc_settings {
   int32_t refresh_period;  /* FP32 path: FP64 residual refresh every this
                               many epochs of ceil(n/tau) steps inside
                               acdc_run_updates (0 = off) */
-  int32_t launch_mode;     /* acdc_launch_mode: how tau > 1 steps execute */
+  int32_t launch_mode;     /* acdc_launch_mode: how the steps execute */
 } acdc_settings;
 
-/* Execution of synchronous steps with tau > 1; the arithmetic is the same. */
+/* Execution of synchronous steps with tau > 1 (modes 0-3), or of
+ * asynchronous workers (modes 0, 4, 5); the arithmetic is the same. */
 typedef enum acdc_launch_mode {
   ACDC_LAUNCH_AUTO = 0,      /* persistent cooperative grid when all tau slots
                                 fit on the device at once, else per step,
@@ -110,7 +111,15 @@ typedef enum acdc_launch_mode {
                                 n <= 65535 and n * sizeof(T) <= 96 KiB (FP32:
                                 n <= 24576, FP64: n <= 12288).  Same
                                 arithmetic as ACDC_LAUNCH_PER_STEP up to the
-                                order of the additions into the residual */
+                                order of the additions into the residual */,
+  /* asynchronous workers (ACDC_STEP_ASYNC), same semantics either way: */
+  ACDC_LAUNCH_ASYNC_DIRECT = 4, /* each worker loads its own records and
+                                entries (any lanes_per_coord) */
+  ACDC_LAUNCH_ASYNC_STAGED = 5  /* one worker per 64-lane wave; a loader wave
+                                per 8 workers copies the coming updates'
+                                records and entries into LDS ahead of use,
+                                so a worker waits only on its residual
+                                gathers (lanes_per_coord 0 or 64) */
 } acdc_launch_mode;
 
 /* One evaluation, all in FP64.  Counterparts: SquareLoss::getResultLogRow
@@ -184,7 +193,10 @@ acdc_status acdc_solver_destroy(acdc_solver *s);
  * (MulticoreEngineExecutor.h:83-89) with innerIterations = n_updates.
  * Returns once the work is queued on the solver's stream (persistent
  * launches: once it has run), so consecutive calls run back to back; the
- * report, the getters and acdc_synchronize wait for it. */
+ * report, the getters and acdc_synchronize wait for it.  A device error in
+ * queued updates is therefore returned by the next call that waits for the
+ * stream (acdc_synchronize, acdc_report_get, acdc_get_x, ...), under that
+ * call's name. */
 acdc_status acdc_run_updates(acdc_solver *s, uint64_t n_updates);
 
 /* r = Ax - b in FP64 (SVM: w = (1/(lambda n)) sum alpha_i y_i x_i).
@@ -277,14 +289,33 @@ typedef struct acdc_batch_settings {
                               instance, from 20 power steps on its normalised
                               Gram matrix at create, as acdc_solver_create
                               does: mSDCA.cpp:188-286) */
+  int32_t screening;       /* Lasso instances with asynchronous workers: 1 =
+                              gap-safe working sets (at every full FP64
+                              refresh, coordinates j with s|<a_j,r>| +
+                              ||a_j|| sqrt(2 gap) < lambda, s = min(1,
+                              lambda/||A^T r||_inf), are provably zero at
+                              the optimum and are set aside; the workers draw
+                              uniformly from the rest, with its own ESO beta
+                              (omega rule), and cheap refreshes over it check
+                              its gap; a full refresh confirms the whole
+                              problem's gap before a pair counts as
+                              converged).  0 = every coordinate throughout.
+                              The optimum is the same; the coordinate
+                              sequence is not. */
 } acdc_batch_settings;
 
 typedef struct acdc_batch_result {
   double lambda, primal, dual, rel_gap, nnz_x;
-  int32_t epochs, converged;
+  int32_t epochs, converged; /* epochs: updates of this lambda over n, rounded up */
   uint64_t updates; /* coordinate updates of the instance up to this lambda */
   double device_us; /* device time of this lambda (refreshes included) */
+  double ws_size;   /* coordinates the workers drew from last (n without screening) */
+  int32_t status;   /* 0; ACDC_BATCH_NONFINITE: an evaluation of the instance
+                       held a NaN or an infinity (data or iterate), so it was
+                       abandoned at this lambda and the rest of its path;
+                       the other instances are unaffected */
 } acdc_batch_result;
+#define ACDC_BATCH_NONFINITE 1
 
 acdc_status acdc_batch_create(const acdc_problem *const *problems, int32_t count,
                               const double *lambdas /* count x path_len */,

@@ -159,3 +159,29 @@ def test_async_warm_start_and_reports(acdc, orc, c1):
     e = orc.CD(0, c1.m, c1.n, c1.ptr, c1.idx, c1.vals, c1.rhs, c1.lam).eval(s.x)
     assert e["rel_gap"] <= 1e-6
     s.close()
+
+
+def test_async_fp32_svm_set_lambda_keeps_alpha(acdc, orc):
+    """FP32 asynchronous SVM: run_updates with no refresh, set_lambda, more
+    updates.  The live alpha is in the workers' records when set_lambda
+    rescales w; the running w afterwards must still equal
+    A^T (alpha y) / (lambda n) for the device's alpha (FP32 corrections:
+    1e-4 of |w|), and the oracle's gap of that alpha matches the report."""
+    d = acdc.gen_svm(1500, 30_000, row_median=100.0, row_sigma=0.9, wstar_nnz=500,
+                     flip_frac=0.1, seed=4)
+    p = d.problem()
+    s = acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=d.lam, tau=32, seed=7,
+                    step_mode=acdc.STEP_ASYNC, precision=acdc.PREC_FP32_REFRESH,
+                    refresh_period=0)
+    s.run_updates(3 * d.n + 5)
+    s.set_lambda(0.5 * d.lam)
+    s.run_updates(3 * d.n)
+    w_dev = s.residuals
+    a = s.x
+    o = orc.SVM(d.n, d.m, d.ptr, d.idx, d.vals, d.rhs, 0.5 * d.lam)
+    o.alpha[:] = a
+    o.recompute_w()
+    assert float(np.max(np.abs(w_dev - o.w))) <= 1e-4 * float(np.max(np.abs(o.w)))
+    rep = s.report()
+    assert rep["gap"] == pytest.approx(o.eval()["gap"], rel=1e-9, abs=1e-12)
+    s.close()
