@@ -123,6 +123,7 @@ acdc_status enqueue_run(acdc_batch *b) {
   a.chunk = std::max(1, b->set.queue_chunk);
   a.results = b->dres;
   a.lds_doubles = (int32_t)(kLdsBytes / 8);
+  a.screening = b->set.screening;
   void *args[] = {&a};
   const size_t shmem = (kLdsBytes / tsz) * tsz;
   BHIP(hipLaunchKernel(batch_kernel(b), dim3(std::min(b->blocks, b->count)), dim3(kBatchBlock),
@@ -158,6 +159,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: lanes_per_coord must be 0, 8, 16, 32 or 64");
   if (set->beta_mode != ACDC_BETA_OMEGA && set->beta_mode != ACDC_BETA_SPECTRAL)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: beta_mode must be omega or spectral");
+  if (set->screening != 0 && set->screening != 1)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: screening must be 0 or 1");
   if (set->loss != ACDC_LOSS_L1_SQUARE && set->loss != ACDC_LOSS_NNLS &&
       set->loss != ACDC_LOSS_SVM_DUAL_HINGE)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: loss must be L1-square, NNLS or SVM dual");
@@ -230,6 +233,12 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   double *db, *dr64;
   acdc_status st;
   if ((st = balloc(b, &b->dcs, tot_n)) != ACDC_OK) return bail(st);
+  // working sets (screened Lasso instances with asynchronous workers only)
+  const bool ws = set->screening && !svm && set->loss == ACDC_LOSS_L1_SQUARE && b->async;
+  CoordRec *dws = nullptr;
+  int32_t *dwsi = nullptr;
+  if (ws && ((st = balloc(b, &dws, tot_n)) != ACDC_OK || (st = balloc(b, &dwsi, tot_n)) != ACDC_OK))
+    return bail(st);
   // one pad entry after the last segment: workers load a padding lane's
   // entry unconditionally (from the segment start, which for an empty last
   // segment is the pad)
@@ -260,6 +269,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     const acdc_problem *p = problems[i];
     BatchInstance &I = hinst[i];
     I.cs = b->dcs + on;
+    I.ws = ws ? dws + on : nullptr;
+    I.wsi = ws ? dwsi + on : nullptr;
     I.ent = dent + onnz * esz;
     I.rent = drent + onnz;
     I.nnz = (int32_t)p->nnz;
@@ -302,6 +313,7 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
       if (sv) acdc_solver_destroy(sv);
       if (bs != ACDC_OK) return bail(bs);
     }
+    I.beta = beta;
     b->n.push_back(p->ncoord);
     b->m.push_back(p->nres);
     b->xoff.push_back(on);
@@ -433,6 +445,8 @@ extern "C" acdc_status acdc_batch_results(acdc_batch *b, acdc_batch_result *out,
     out[k].converged = r[k].converged;
     out[k].updates = r[k].updates;
     out[k].device_us = (double)r[k].ticks * 1e3 / b->clock_khz;
+    out[k].ws_size = r[k].ws_size;
+    out[k].status = r[k].status;
   }
   return ACDC_OK;
 }

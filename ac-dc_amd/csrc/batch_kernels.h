@@ -49,6 +49,9 @@ struct alignas(16) RefEntry {
 
 struct BatchInstance {
   CoordRec *cs;          // n records
+  CoordRec *ws;          // working set: copies of the records of the coordinates
+                         // gap-safe screening keeps (Lasso, screening on), n slots
+  int32_t *wsi;          // coordinate of each working-set slot
   const void *ent;       // Entry<T>[nnz]
   const RefEntry *rent;  // FP64 entries of the refresh, entry order
   const double *b;       // null for SVM instances (the residual is w = A^T (alpha y) / (lambda n))
@@ -62,6 +65,7 @@ struct BatchInstance {
                          // coordinates are samples, x is alpha, CoordRec.L is
                          // y_j / (beta ||x_j||^2): its sign is the label)
   double scale;          // NNLS: ||A^T b||_inf, the KKT residual's scale
+  double beta;           // ESO beta of the records' L_j = 1/(beta ||a_j||^2)
   uint64_t seed;
 };
 
@@ -117,8 +121,18 @@ struct BatchResult {
   int32_t epochs, converged;
   uint64_t updates;
   uint64_t ticks;  // wall-clock ticks of this lambda
+  double grad_inf;   // max_j |<a_j, r>| of the evaluation
+  double ws_size;    // coordinates in the working set at the end (n without screening)
+  int32_t status;    // 0, or kBatchNonFinite
 };
 
+// An evaluation with a NaN or an infinity (in the data or the iterate)
+// abandons the instance: this and its remaining lambdas are reported with
+// this status, and its block takes the next instance from the queue.
+constexpr int32_t kBatchNonFinite = 1;
+__device__ __forceinline__ bool finite_eval(const BatchResult &r) {
+  return isfinite(r.primal) && isfinite(r.dual) && isfinite(r.rel_gap);
+}
 struct BatchArgs {
   const BatchInstance *inst;
   const int32_t *order;  // instances of this launch, in processing order
@@ -132,7 +146,29 @@ struct BatchArgs {
   int32_t chunk;         // queue items a block takes per queue access
   BatchResult *results;  // instances x path_len
   int32_t lds_doubles;   // FP64 rows the dynamic LDS holds (refresh residual)
+  int32_t screening;     // Lasso, asynchronous workers: gap-safe working sets
 };
+
+// the results of lambdas l .. path_len - 1 of an abandoned instance
+__device__ __forceinline__ void batch_fail_rest(const BatchArgs &a, int i, int l,
+                                                const BatchResult &last, uint64_t updates) {
+  if (threadIdx.x != 0) return;
+  for (int k = l; k < a.path_len; ++k) {
+    BatchResult *o = a.results + (int64_t)i * a.path_len + k;
+    o->lambda = a.lambdas[(int64_t)i * a.path_len + k];
+    o->primal = last.primal;
+    o->dual = last.dual;
+    o->rel_gap = last.rel_gap;
+    o->nnz_x = last.nnz_x;
+    o->epochs = 0;
+    o->converged = 0;
+    o->updates = updates;
+    o->ticks = 0;
+    o->grad_inf = last.grad_inf;
+    o->ws_size = 0.0;
+    o->status = kBatchNonFinite;
+  }
+}
 
 // zeroes x of every coordinate record and the work-queue counters (a kernel
 // rather than a memset so that the whole run can live in one captured graph)
@@ -151,7 +187,7 @@ constexpr int kBatchWaves = kBatchBlock / 64;
 // Statistics of a refresh reduced over the block in a fixed order: an xor
 // butterfly inside each wave, then thread 0 folds the 16 wave partials in
 // index order.  Only thread 0 holds the totals.
-constexpr int kRefreshStats = 5;  // max|g|, ||x||_1, nnz(x), ||r||^2, b.r
+constexpr int kRefreshStats = 6;  // max|g|, ||x||_1, nnz(x), ||r||^2, b.r, non-finite g
 __device__ __forceinline__ void block_stats(double (&v)[kRefreshStats], double *sh) {
   v[0] = group_max<64>(v[0]);
 #pragma unroll
@@ -291,6 +327,7 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
   __syncthreads();
   // r += x_j a_j over the nonzero x_j.  ||x||_1 and nnz(x) are counted here.
   double ginf = 0, x1 = 0, nz = 0;
+  double bad = 0;  // gradients that are NaN or infinite (fmax drops NaN)
   const int wv = tid >> 6, ln = tid & 63;
   // wave w's contiguous share of the entries (both streaming passes)
   const int32_t e_lo = (int32_t)((int64_t)I.nnz * wv / kBatchWaves);
@@ -398,6 +435,7 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
       } else {
         *gp(&I.cs[c].gref) = v;
         ginf = fmax(ginf, fabs(v));
+        bad += isfinite(v) ? 0.0 : 1.0;
       }
     };
     RefEntry nxt[U];
@@ -465,6 +503,7 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
         if (c >= 0) {
           *gp(&I.cs[c].gref) = v;
           ginf = fmax(ginf, fabs(v));
+          bad += isfinite(v) ? 0.0 : 1.0;
         }
         c = kc;
         v = sh[2 * k + 1];
@@ -472,6 +511,7 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
       if (c >= 0) {
         *gp(&I.cs[c].gref) = v;
         ginf = fmax(ginf, fabs(v));
+        bad += isfinite(v) ? 0.0 : 1.0;
       }
     }
     __syncthreads();  // sh is free for block_stats
@@ -492,10 +532,11 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
       const double h = 1.0 - sign_of(gld(&I.cs[j].L)) * gld(&I.cs[j].gref);
       br += h > 0.0 ? h : 0.0;
     }
-  double st[kRefreshStats] = {ginf, x1, nz, r2, br};
+  double st[kRefreshStats] = {ginf, x1, nz, r2, br, bad};
   block_stats(st, sh);  // its barrier also ends every read of r
   dr_view<T, LDSR>(I, lds).zero(I.m);
   if (tid == 0) {
+    out->grad_inf = st[0];
     if (!svm && I.loss == kNnls) {
       // objective 1/2 ||Ax - b||^2 (no dual is formed: 0); rel_gap is the
       // KKT residual over ||A^T b||_inf
@@ -521,6 +562,7 @@ __device__ __forceinline__ void batch_refresh_impl(const BatchInstance *Ip, doub
       out->nnz_x = st[2];
       out->rel_gap = P != 0 ? (P - D) / fabs(P) : 0.0;
     }
+    if (st[5] > 0.0) out->rel_gap = __builtin_nan("");  // a gradient was not finite
   }
   __syncthreads();  // gref, dr and *out are settled for the steps
 }
@@ -535,6 +577,205 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
     batch_refresh_impl<T, LDSR, LDSR, SVM>(Ip, lambda, lds, sh, out, dense);
   else
     batch_refresh_impl<T, LDSR, false, SVM>(Ip, lambda, lds, sh, out, dense);
+}
+
+// ---------------------------------------------------- gap-safe working sets
+// Gap-safe screening (Fercoq, Gramfort and Salmon, "Mind the duality gap:
+// safer rules for the Lasso", ICML 2015), used by the asynchronous Lasso
+// instances.  The dual D(u) = b^T u - ||u||^2 / 2 (u = -s r, s = min(1,
+// lambda / ||A^T r||_inf), the dual point of every evaluation here) is
+// 1-strongly concave, so the dual optimum u* lies within sqrt(2 G) of u for
+// the duality gap G = P(x) - D(u), and |<a_j, u*>| <= s |g_j| + ||a_j||
+// sqrt(2 G) with g_j = <a_j, r>.  Where that bound is below lambda, x*_j = 0
+// (the optimality condition of a nonzero x*_j is |<a_j, u*>| = lambda), so
+// coordinate j can be left out without changing the optimum.  The screened
+// problem is solved by the same asynchronous workers, now drawing uniformly
+// from the working set W (its records copied into I.ws in coordinate order),
+// with the ESO beta of W: 1 + (omega_W - 1)(tau_W - 1)/(|W| - 1), omega_W the
+// most entries of W in one row, tau_W = min(tau, |W|) workers.
+
+// screens with the full refresh's g_j (in I.cs) and gap; copies the records
+// of W into I.ws and their coordinates into I.wsi; returns |W|.  Coordinates
+// with x_j != 0 stay in W.  sh: at least kBatchWaves ints of scratch.
+__device__ __forceinline__ int32_t batch_screen(const BatchInstance &I, double lambda,
+                                                double gap_abs, double ginf, int *shi) {
+  const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
+  const double s = ginf > lambda ? lambda / ginf : 1.0;
+  // a relative margin of 1e-9 on the radius keeps rounding from screening a
+  // coordinate on the boundary
+  const double rad = sqrt(fmax(2.0 * gap_abs, 0.0)) * (1.0 + 1e-9) + 1e-15 * lambda;
+  int32_t base = 0;
+  for (int32_t j0 = 0; j0 < I.n; j0 += kBatchBlock) {
+    const int32_t j = j0 + tid;
+    bool keep = false;
+    CoordRec r{};
+    if (j < I.n) {
+      r = gld(I.cs + j);
+      r.x = gld_agent(&I.cs[j].x);  // workers add to x atomically
+      const double nrm = r.L > 0.0 ? sqrt(1.0 / (I.beta * r.L)) : 0.0;
+      keep = r.x != 0.0 || (r.L > 0.0 && s * fabs(r.gref) + nrm * rad >= lambda);
+    }
+    const uint64_t bal = __ballot(keep);
+    const int within = __popcll(bal & ((1ull << ln) - 1ull));
+    if (ln == 0) shi[wv] = __popcll(bal);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBatchWaves; ++k) {
+      const int c = shi[k];
+      off += k < wv ? c : 0;
+      tot += c;
+    }
+    if (keep) {
+      I.ws[base + off + within] = r;
+      *gp(I.wsi + base + off + within) = j;
+    }
+    base += tot;
+    __syncthreads();  // shi is rewritten by the next chunk
+  }
+  return base;
+}
+
+// omega_W: the most entries one row holds among the working set's columns,
+// counted into the correction residual's storage (LDS rows below the split,
+// the instance's global rows above it) and zeroed again for the updates.
+template <typename T, bool LDSR>
+__device__ __forceinline__ int32_t batch_ws_omega(const BatchInstance &I, int32_t nw, T *lds,
+                                                  int *shi) {
+  const int tid = threadIdx.x;
+  const int32_t split = LDSR ? I.m : I.lds_rows;
+  int32_t *cl = reinterpret_cast<int32_t *>(lds);
+  int32_t *cg = reinterpret_cast<int32_t *>(I.drg);
+  for (int32_t k = tid; k < I.m; k += kBatchBlock) {
+    if (k < split)
+      cl[k] = 0;
+    else
+      *gp(cg + (k - split)) = 0;
+  }
+  __syncthreads();
+  constexpr int G = 16;  // lanes per column
+  const int grp = tid / G, lane = tid & (G - 1);
+  for (int32_t u = grp; u < nw; u += kBatchBlock / G) {
+    const int32_t p0 = gld(&I.ws[u].p0), len = gld(&I.ws[u].len);
+    for (int32_t q = lane; q < len; q += G) {
+      const int32_t row = gld(I.rent + p0 + q).row;
+      if (row < split)
+        atomicAdd(cl + row, 1);
+      else
+        __hip_atomic_fetch_add(gp(cg + (row - split)), 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __syncthreads();
+  int mx = 1;  // counted from 1, as the reference's omega (MultiCoreSolver.cpp:24)
+  for (int32_t k = tid; k < I.m; k += kBatchBlock)
+    mx = max(mx, k < split ? cl[k] : gld_agent(cg + (k - split)));
+  mx = group_max<64>(mx);
+  if ((tid & 63) == 0) shi[tid >> 6] = mx;
+  __syncthreads();
+  int om = shi[0];
+  for (int k = 1; k < kBatchWaves; ++k) om = max(om, shi[k]);
+  __syncthreads();
+  dr_view<T, LDSR>(I, lds).zero(I.m);
+  __syncthreads();
+  return om;
+}
+
+// FP64 refresh restricted to the working set: r64 = A x - b from W's nonzero
+// x (every other x_j is zero), g_u = <a_j, r64> into I.ws[u].gref, and the
+// Lasso primal, dual (dual point scaled by max_u |g_u|, the dual of the
+// screened problem) and relative gap into *out.  Zeroes dr.
+template <typename T, bool LDSR, bool RL>
+__device__ __forceinline__ void batch_refresh_ws(const BatchInstance &I, int32_t nw,
+                                                 double lambda, T *lds, double *sh,
+                                                 BatchResult *out) {
+  const int tid = threadIdx.x;
+  double *rs = reinterpret_cast<double *>(lds);
+  double *rg = I.r64;
+  auto rget = [&](int32_t row) -> double { return RL ? rs[row] : gld_agent(rg + row); };
+  auto radd = [&](int32_t row, double v) {
+    if (RL)
+      atomicAdd(rs + row, v);
+    else
+      gadd_block(rg + row, v);
+  };
+  for (int32_t k = tid; k < I.m; k += kBatchBlock) {
+    const double v = -gld(I.b + k);
+    if (RL)
+      rs[k] = v;
+    else
+      *gp(rg + k) = v;
+  }
+  __syncthreads();
+  constexpr int G = 16;  // lanes per column
+  const int grp = tid / G, lane = tid & (G - 1);
+  double x1 = 0, nz = 0, ginf = 0, bad = 0;
+  for (int32_t u = grp; u < nw; u += kBatchBlock / G) {
+    const double xv = gld_agent(&I.ws[u].x);
+    if (xv == 0.0) continue;  // uniform in the group
+    if (lane == 0) {
+      x1 += fabs(xv);
+      nz += 1.0;
+    }
+    const int32_t p0 = gld(&I.ws[u].p0), len = gld(&I.ws[u].len);
+    for (int32_t q0 = lane; q0 < len; q0 += 4 * G) {
+      RefEntry e[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) e[k] = gld(I.rent + p0 + min(q0 + k * G, len - 1));
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (q0 + k * G < len) radd(e[k].row, e[k].val * xv);
+    }
+  }
+  __syncthreads();
+  for (int32_t u = grp; u < nw; u += kBatchBlock / G) {
+    const int32_t p0 = gld(&I.ws[u].p0), len = gld(&I.ws[u].len);
+    double g = 0.0;
+    for (int32_t q0 = lane; q0 < len; q0 += 4 * G) {
+      RefEntry e[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) e[k] = gld(I.rent + p0 + min(q0 + k * G, len - 1));
+      double rv[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) rv[k] = rget(e[k].row);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (q0 + k * G < len) g += e[k].val * rv[k];
+    }
+    g = group_sum<G>(g);
+    if (lane == 0) {
+      *gp(&I.ws[u].gref) = g;
+      ginf = fmax(ginf, fabs(g));
+      bad += isfinite(g) ? 0.0 : 1.0;
+    }
+  }
+  double r2 = 0, br = 0;
+  for (int32_t k = tid; k < I.m; k += kBatchBlock) {
+    const double rk = rget(k);
+    r2 += rk * rk;
+    br += gld(I.b + k) * rk;
+  }
+  double st[kRefreshStats] = {ginf, x1, nz, r2, br, bad};
+  block_stats(st, sh);
+  dr_view<T, LDSR>(I, lds).zero(I.m);
+  if (tid == 0) {
+    const double P = 0.5 * st[3] + lambda * st[1];
+    const double sc = st[0] > lambda ? lambda / st[0] : 1.0;
+    const double D = -sc * st[4] - 0.5 * sc * sc * st[3];
+    out->grad_inf = st[0];
+    out->primal = P;
+    out->dual = D;
+    out->nnz_x = st[2];
+    out->rel_gap = st[5] > 0.0 ? __builtin_nan("") : (P != 0 ? (P - D) / fabs(P) : 0.0);
+  }
+  __syncthreads();
+}
+
+// x of the working set back into the instance's records
+__device__ __forceinline__ void batch_ws_writeback(const BatchInstance &I, int32_t nw) {
+  for (int32_t u = threadIdx.x; u < nw; u += kBatchBlock)
+    *gp(&I.cs[gld(I.wsi + u)].x) = gld_agent(&I.ws[u].x);
+  __syncthreads();
 }
 
 // nsteps synchronous steps of one instance; S slots per G-lane group.  The
@@ -656,8 +897,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
       const double lambda = a.lambdas[(int64_t)i * a.path_len + l];
       refresh(lambda);
       double gap = s_res.rel_gap;
+      bool ok = finite_eval(s_res);
       int epochs = 0;
-      while (gap > a.gap_tol && epochs < a.max_epochs) {
+      while (ok && gap > a.gap_tol && epochs < a.max_epochs) {
         const int ep = min(a.refresh_period, a.max_epochs - epochs);
         const uint64_t ns = steps_per_epoch * (uint64_t)ep;
         if (in_lds)
@@ -668,6 +910,12 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
         epochs += ep;
         refresh(lambda);
         gap = s_res.rel_gap;
+        ok = finite_eval(s_res);
+      }
+      if (!ok) {  // NaN or infinity: abandon the instance
+        batch_fail_rest(a, i, l, s_res, step * (uint64_t)I.tau);
+        __syncthreads();
+        break;
       }
       if (threadIdx.x == 0) {
         const uint64_t ticks = wall_clock64() - t0;
@@ -681,6 +929,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_lasso(BatchArgs a) {
         o->converged = gap <= a.gap_tol;
         o->updates = step * (uint64_t)I.tau;
         o->ticks = ticks;
+        o->grad_inf = s_res.grad_inf;
+        o->ws_size = (double)I.n;
+        o->status = 0;
       }
       __syncthreads();  // s_res is rewritten by the next refresh
     }
@@ -855,6 +1106,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
     const BatchInstance *Ip = a.inst + i;
     const BatchInstance I = *Ip;
     const bool in_lds = I.drg == nullptr;
+    double lambda_of_item = 0.0;
     // the scatter streams every entry once the last refresh of this
     // instance found more than n/64 nonzero x_j (where streaming all entries
     // costs less than the columns' dependent loads, measured)
@@ -866,25 +1118,98 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
         batch_refresh<T, false, SVM>(Ip, lambda, lds, sh, a.lds_doubles, &s_res, dense);
       dense = 64.0 * s_res.nnz_x > (double)I.n;
     };
-    uint64_t k = 0;  // updates per worker so far
+    auto updates = [&](const BatchInstance &J, uint64_t k0, uint64_t per) {
+      if (in_lds)
+        batch_async_updates<T, G, R, true, TAIL, SVM>(J, lambda_of_item, k0, per, lds);
+      else
+        batch_async_updates<T, G, R, false, TAIL, SVM>(J, lambda_of_item, k0, per, lds);
+      __syncthreads();  // every worker's updates are in before the refresh
+    };
+    // Lasso instances with screening: gap-safe working sets
+    const bool screen = !SVM && a.screening && I.loss == kLasso;
+    uint64_t k = 0;      // updates per worker so far (the workers' stream offset)
+    uint64_t total = 0;  // updates of the instance so far
     for (int l = 0; l < a.path_len; ++l) {
       const uint64_t t0 = wall_clock64();
       const double lambda = a.lambdas[(int64_t)i * a.path_len + l];
+      lambda_of_item = lambda;
       refresh(lambda);
       double gap = s_res.rel_gap;
-      int epochs = 0;
-      while (gap > a.gap_tol && epochs < a.max_epochs) {
-        const int ep = min(a.refresh_period, a.max_epochs - epochs);
-        const uint64_t per = ((uint64_t)I.n * (uint64_t)ep + I.tau - 1) / I.tau;
-        if (in_lds)
-          batch_async_updates<T, G, R, true, TAIL, SVM>(I, lambda, k, per, lds);
-        else
-          batch_async_updates<T, G, R, false, TAIL, SVM>(I, lambda, k, per, lds);
-        __syncthreads();  // every worker's updates are in before the refresh
-        k += per;
-        epochs += ep;
-        refresh(lambda);
-        gap = s_res.rel_gap;
+      bool ok = finite_eval(s_res);
+      // updates of this lambda, against the cap of max_epochs epochs of n
+      const double cap = (double)a.max_epochs * (double)I.n;
+      double done = 0.0;
+      int32_t nw = I.n;          // working-set size (n: all coordinates)
+      int32_t nw_last = I.n;     // the size of the last set the workers drew from
+      double tol_w = a.gap_tol;  // the working set's gap target
+      while (ok && gap > a.gap_tol && done < cap) {
+        int32_t w_count = 0;
+        if (screen) {
+          // the full refresh just run holds g = A^T r and the gap
+          w_count = batch_screen(I, lambda, s_res.primal - s_res.dual, s_res.grad_inf,
+                                 reinterpret_cast<int *>(sh));
+        }
+        if (screen && w_count >= 1 && 10 * (int64_t)w_count <= 9 * (int64_t)I.n) {
+          // the screened problem: uniform draws from W with W's ESO beta
+          nw = w_count;
+          nw_last = nw;
+          const int32_t om = in_lds ? batch_ws_omega<T, true>(I, nw, lds, reinterpret_cast<int *>(sh))
+                                    : batch_ws_omega<T, false>(I, nw, lds, reinterpret_cast<int *>(sh));
+          BatchInstance J = I;
+          J.cs = I.ws;
+          J.n = nw;
+          J.tau = min(I.tau, nw);
+          const double bw = nw > 1 ? 1.0 + (double)(om - 1) * (double)(J.tau - 1) / ((double)nw - 1.0)
+                                   : 1.0;
+          for (int32_t u = threadIdx.x; u < nw; u += kBatchBlock)
+            *gp(&I.ws[u].L) = gld(&I.ws[u].L) * (I.beta / bw);
+          __syncthreads();
+          const double screen_gap = gap;
+          double gap_w = gap;
+          while (done < cap) {
+            const uint64_t per =
+                ((uint64_t)nw * (uint64_t)a.refresh_period + J.tau - 1) / J.tau;
+            updates(J, k, per);
+            k += per;
+            done += (double)per * J.tau;
+            total += per * (uint64_t)J.tau;
+            if (in_lds ? I.m <= a.lds_doubles : false)
+              batch_refresh_ws<T, true, true>(J, nw, lambda, lds, sh, &s_res);
+            else if (in_lds)
+              batch_refresh_ws<T, true, false>(J, nw, lambda, lds, sh, &s_res);
+            else
+              batch_refresh_ws<T, false, false>(J, nw, lambda, lds, sh, &s_res);
+            gap_w = s_res.rel_gap;
+            // back to the whole problem once W's gap meets its target, or
+            // has fallen a hundredfold since the screening (a new screening
+            // then shrinks W), or is not finite (the full refresh below
+            // reports it)
+            if (!finite_eval(s_res) || gap_w <= tol_w || gap_w <= 1e-2 * screen_gap) break;
+          }
+          batch_ws_writeback(I, nw);
+          refresh(lambda);
+          gap = s_res.rel_gap;
+          ok = finite_eval(s_res);
+          // W met its target but the whole problem did not (the dual point
+          // of W is scaled by W's gradients only): aim W lower
+          if (gap > a.gap_tol && gap_w <= tol_w) tol_w *= 0.25;
+          nw = I.n;
+        } else {
+          nw_last = I.n;
+          const uint64_t per = ((uint64_t)I.n * (uint64_t)a.refresh_period + I.tau - 1) / I.tau;
+          updates(I, k, per);
+          k += per;
+          done += (double)per * I.tau;
+          total += per * (uint64_t)I.tau;
+          refresh(lambda);
+          gap = s_res.rel_gap;
+          ok = finite_eval(s_res);
+        }
+      }
+      if (!ok) {  // NaN or infinity: abandon the instance
+        batch_fail_rest(a, i, l, s_res, total);
+        __syncthreads();
+        break;
       }
       if (threadIdx.x == 0) {
         const uint64_t ticks = wall_clock64() - t0;
@@ -894,10 +1219,13 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
         o->dual = s_res.dual;
         o->rel_gap = s_res.rel_gap;
         o->nnz_x = s_res.nnz_x;
-        o->epochs = epochs;
+        o->epochs = (int32_t)ceil(done / (double)I.n);
         o->converged = gap <= a.gap_tol;
-        o->updates = k * (uint64_t)I.tau;
+        o->updates = total;
         o->ticks = ticks;
+        o->grad_inf = s_res.grad_inf;
+        o->ws_size = (double)nw_last;
+        o->status = 0;
       }
       __syncthreads();
     }

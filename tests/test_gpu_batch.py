@@ -149,6 +149,7 @@ def test_batch_async_workers_reach_fp64_optimum(acdc, orc, small_batch, prec):
                    gap_tol=1e-7, use_graph=True, step_mode=acdc.STEP_ASYNC)
     b.run()
     res = b.results()
+    print("epochs", res["epochs"].tolist(), "ws", res["ws_size"].tolist(), flush=True)
     assert res["converged"].all(), res["rel_gap"]
     for i, d in enumerate(ds):
         e = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[i, -1]).eval(b.x(i))
@@ -414,4 +415,33 @@ def test_batch_spectral_beta_svm(acdc, svm_batch):
         assert beta == pytest.approx(s.beta, rel=1e-12)
         s.close()
         assert 1.0 <= beta
+    b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batch_screening_reaches_oracle_gap(acdc, orc, small_batch, prec):
+    """Gap-safe working sets (screening on) for asynchronous Lasso instances
+    along a path down to 1e-3 lambda_max: every pair converges, the oracle's
+    evaluation of each instance's last x has relative gap <= 1e-6 and equals
+    the device's report, and the workers drew from working sets smaller than
+    n along the path."""
+    ds = small_batch
+    lams = lambdas_of(acdc, ds, 8, 1e-3)
+    probs = [d.problem() for d in ds]
+    precision = acdc.PREC_FP32_REFRESH if prec == "fp32" else acdc.PREC_FP64
+    # the serial method needs ~2,600-6,700 epochs of n at the last lambda of
+    # the three wide instances (measured with the oracle on like data)
+    b = acdc.Batch(probs, lams, seed=11, precision=precision, refresh_period=5, max_epochs=40000,
+                   gap_tol=1e-6, use_graph=True, step_mode=acdc.STEP_ASYNC, screening=True)
+    b.run()
+    res = b.results()
+    print("epochs", res["epochs"].tolist(), "ws", res["ws_size"].tolist(), flush=True)
+    assert res["converged"].all(), res["rel_gap"]
+    for i, d in enumerate(ds):
+        e = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[i, -1]).eval(b.x(i))
+        assert e["rel_gap"] <= 1e-6, (i, e["rel_gap"])
+        assert res[i, -1]["primal"] == pytest.approx(e["primal"], rel=1e-9)
+        assert res[i, -1]["rel_gap"] == pytest.approx(e["rel_gap"], rel=1e-6, abs=1e-12)
+    ns = np.array([d.n for d in ds])[:, None]
+    assert (res["ws_size"][:, 1:] < ns).mean() > 0.5, res["ws_size"]
     b.close()
