@@ -445,3 +445,31 @@ def test_batch_screening_reaches_oracle_gap(acdc, orc, small_batch, prec):
     ns = np.array([d.n for d in ds])[:, None]
     assert (res["ws_size"][:, 1:] < ns).mean() > 0.5, res["ws_size"]
     b.close()
+
+
+@pytest.mark.parametrize("mode", ["sync", "async", "async_screened"])
+def test_batch_nonfinite_instance_is_abandoned(acdc, small_batch, mode):
+    """A NaN in one instance's data marks that instance (every lambda of its
+    path) ACDC_BATCH_NONFINITE; the other instances run to their gap target
+    as usual."""
+    ds = small_batch
+    bad = 2
+    probs = []
+    for i, d in enumerate(ds):
+        vals = d.vals.copy()
+        if i == bad:
+            vals[5] = np.nan
+        probs.append(acdc.Problem.from_csc(d.m, d.n, d.ptr, d.idx, vals, d.rhs))
+    lams = lambdas_of(acdc, ds, 3, 0.1)
+    b = acdc.Batch(probs, lams, seed=3, precision=acdc.PREC_FP32_REFRESH, refresh_period=5,
+                   max_epochs=400, gap_tol=1e-6, use_graph=True,
+                   step_mode=acdc.STEP_SYNC if mode == "sync" else acdc.STEP_ASYNC,
+                   screening=mode == "async_screened")
+    b.run()
+    res = b.results()
+    assert (res["status"][bad] == acdc.BATCH_NONFINITE).all()
+    assert not res["converged"][bad].any()
+    good = [i for i in range(len(ds)) if i != bad]
+    assert (res["status"][good] == 0).all()
+    assert res["converged"][good].all(), res["rel_gap"][good]
+    b.close()
