@@ -153,6 +153,8 @@ _SIGS = {
     "acdc_set_lambda": ([_P, ctypes.c_double], ctypes.c_int),
     "acdc_get_beta": ([_P, _F64P], ctypes.c_int),
     "acdc_sample_coordinates": ([_P, ctypes.c_uint64, ctypes.c_uint64, _I32P], ctypes.c_int),
+    "acdc_sample_coordinates_noalloc": ([_P, ctypes.c_uint64, ctypes.c_uint64, _I32P],
+                                        ctypes.c_int),
     "acdc_synchronize": ([_P], ctypes.c_int),
     "acdc_profile_step_kernels": ([_P, ctypes.c_uint64, _F64P, _F64P], ctypes.c_int),
     "acdc_batch_create": ([ctypes.POINTER(_P), ctypes.c_int32, _F64P, ctypes.c_int32,
@@ -408,6 +410,13 @@ class Solver:
     def sample_coordinates(self, step0: int, nsteps: int) -> np.ndarray:
         out = np.empty(nsteps * self.settings.tau, np.int32)
         _check(lib.acdc_sample_coordinates(self._h, step0, nsteps, _ptr(out, ctypes.c_int32)))
+        return out.reshape(nsteps, self.settings.tau)
+
+    def sample_coordinates_noalloc(self, step0: int, nsteps: int) -> np.ndarray:
+        """sample_coordinates through the allocation-free entry point."""
+        out = np.empty(nsteps * self.settings.tau, np.int32)
+        _check(lib.acdc_sample_coordinates_noalloc(self._h, step0, nsteps,
+                                                   _ptr(out, ctypes.c_int32)))
         return out.reshape(nsteps, self.settings.tau)
 
     def profile_step_kernels(self, nsteps: int) -> tuple:

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -213,6 +214,7 @@ struct acdc_solver {
   int64_t maxlen = 0;  // longest segment
   int64_t coresident_blocks = 0;
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current at create (the stream's)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t *ptr = nullptr, *idx = nullptr;
   void *val = nullptr;
@@ -786,6 +788,8 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
     return e;
   };
   acdc_status st = ACDC_OK;
+  if (hipGetDevice(&s->device) != hipSuccess)
+    return bail(fail(ACDC_ERR_HIP, "acdc_solver_create: no current device"));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return bail(fail(ACDC_ERR_HIP, "acdc_solver_create: stream/event creation failed"));
@@ -1254,6 +1258,65 @@ extern "C" acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0, u
   (void)hipFree(d);
   if (e != hipSuccess)
     return fail(ACDC_ERR_HIP, std::string("acdc_sample_coordinates: ") + hipGetErrorString(e));
+  return ACDC_OK;
+}
+
+// ---- allocation-free sampling: the staging array is a file-scope device
+// array (one copy per device), whose address on each device is looked up
+// once with hipGetSymbolAddress while that device is current; host code uses
+// only that address, never the array's name.
+static __device__ __align__(256) int32_t sample_scratch[ACDC_NOALLOC_MAX_SAMPLES];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch[kMaxDevices] = {};  // per-device address of sample_scratch
+
+// the address of sample_scratch on `device` (made current for the lookup)
+acdc_status scratch_on(int device, int32_t **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(ACDC_ERR_INVALID, "acdc_sample_coordinates_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    int prev = 0;
+    HIP_TRY(hipGetDevice(&prev));
+    HIP_TRY(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(sample_scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess || !p)
+      return fail(ACDC_ERR_HIP, std::string("acdc_sample_coordinates_noalloc: hipGetSymbolAddress: ") +
+                                    hipGetErrorString(e));
+    g_scratch[device] = static_cast<int32_t *>(p);
+  }
+  *out = g_scratch[device];
+  return ACDC_OK;
+}
+}  // namespace
+
+extern "C" acdc_status acdc_sample_coordinates_noalloc(acdc_solver *s, uint64_t step0,
+                                                       uint64_t nsteps, int32_t *out) {
+  if (!s || !out) return fail(ACDC_ERR_INVALID, "acdc_sample_coordinates_noalloc: null argument");
+  const uint64_t tau = (uint64_t)s->set.tau;
+  if (nsteps > (uint64_t)ACDC_NOALLOC_MAX_SAMPLES / tau)
+    return fail(ACDC_ERR_INVALID, "acdc_sample_coordinates_noalloc: nsteps * tau exceeds " +
+                                      std::to_string(ACDC_NOALLOC_MAX_SAMPLES) + " samples");
+  const int64_t total = (int64_t)(nsteps * tau);
+  if (total == 0) return ACDC_OK;
+  int32_t *d = nullptr;
+  acdc_status st = scratch_on(s->device, &d);
+  if (st != ACDC_OK) return st;
+  // the kernels write every entry of [0, total) before the copy reads it
+  if (s->async)
+    k_sample_async<<<(unsigned)div_up(total, kBlock), kBlock, 0, s->stream>>>(
+        (uint32_t)s->ncoord, s->set.seed, step0, nsteps, s->set.tau, d);
+  else
+    k_sample<<<(unsigned)div_up(total, kBlock), kBlock, 0, s->stream>>>(
+        Perm::make((uint32_t)s->ncoord), s->set.seed, step0, nsteps, s->set.tau, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)total * 4, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess)
+    return fail(ACDC_ERR_HIP, std::string("acdc_sample_coordinates_noalloc: ") + hipGetErrorString(e));
   return ACDC_OK;
 }
 

@@ -242,6 +242,19 @@ acdc_status acdc_sample_coordinates(acdc_solver *s, uint64_t step0,
 acdc_status acdc_profile_step_kernels(acdc_solver *s, uint64_t nsteps,
                                       double *gather_ms, double *scatter_ms);
 
+/* acdc_sample_coordinates without any allocation: the same coordinates,
+ * staged in a fixed device array of ACDC_NOALLOC_MAX_SAMPLES int32 entries
+ * (4 MiB, held by each device the library runs on until the process ends)
+ * instead of a buffer allocated per call.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  nsteps * tau
+ * above the maximum is ACDC_ERR_INVALID.  Every call on one device shares
+ * the array, so calls that overlap on one device (other host threads, other
+ * solvers' streams) race; calls on one solver are serialised by the caller. */
+#define ACDC_NOALLOC_MAX_SAMPLES (1 << 20)
+acdc_status acdc_sample_coordinates_noalloc(acdc_solver *s, uint64_t step0, uint64_t nsteps,
+                                            int32_t *out);
+
 /* Waits for all work queued on the solver's stream. */
 acdc_status acdc_synchronize(acdc_solver *s);
 
