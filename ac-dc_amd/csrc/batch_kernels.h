@@ -579,26 +579,34 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
     batch_refresh_impl<T, LDSR, false, SVM>(Ip, lambda, lds, sh, out, dense);
 }
 
-// ---------------------------------------------------- gap-safe working sets
+// ------------------------------------------------------------- working sets
+// The asynchronous Lasso instances of a screened batch solve each lambda on
+// a working set W of coordinates, re-formed at every full FP64 refresh from
+// g = A^T r of the current x:
+//   W = {j : x_j != 0} u {j : |g_j| >= thr} minus the coordinates gap-safe
+//   screening proves zero,
+// with thr = min(lambda, 2 lambda - lambda_prev), the sequential strong rule
+// of Tibshirani et al. (JRSS-B 2012) for the step from the previous lambda
+// of the path.  Since thr <= lambda, every coordinate that violates the
+// optimality conditions at x (|g_j| > lambda with x_j = 0) is in W, so when
+// the workers have solved the problem restricted to W, the full refresh
+// either confirms the whole problem's gap or brings the violators in.
 // Gap-safe screening (Fercoq, Gramfort and Salmon, "Mind the duality gap:
-// safer rules for the Lasso", ICML 2015), used by the asynchronous Lasso
-// instances.  The dual D(u) = b^T u - ||u||^2 / 2 (u = -s r, s = min(1,
-// lambda / ||A^T r||_inf), the dual point of every evaluation here) is
-// 1-strongly concave, so the dual optimum u* lies within sqrt(2 G) of u for
-// the duality gap G = P(x) - D(u), and |<a_j, u*>| <= s |g_j| + ||a_j||
-// sqrt(2 G) with g_j = <a_j, r>.  Where that bound is below lambda, x*_j = 0
-// (the optimality condition of a nonzero x*_j is |<a_j, u*>| = lambda), so
-// coordinate j can be left out without changing the optimum.  The screened
-// problem is solved by the same asynchronous workers, now drawing uniformly
-// from the working set W (its records copied into I.ws in coordinate order),
-// with the ESO beta of W: 1 + (omega_W - 1)(tau_W - 1)/(|W| - 1), omega_W the
-// most entries of W in one row, tau_W = min(tau, |W|) workers.
+// safer rules for the Lasso", ICML 2015): the dual D(u) = b^T u - ||u||^2/2
+// (u = -s r, s = min(1, lambda/||A^T r||_inf), the dual point of every
+// evaluation here) is 1-strongly concave, so the dual optimum u* lies within
+// sqrt(2 G) of u for the duality gap G = P(x) - D(u), and |<a_j, u*>| <=
+// s |g_j| + ||a_j|| sqrt(2 G); below lambda, x*_j = 0 (a nonzero x*_j needs
+// |<a_j, u*>| = lambda).  The workers draw uniformly from W, with the ESO
+// beta of W: 1 + (omega_W - 1)(tau_W - 1)/(|W| - 1), omega_W the most
+// entries of W in one row, tau_W = min(tau, |W|) workers.
 
-// screens with the full refresh's g_j (in I.cs) and gap; copies the records
-// of W into I.ws and their coordinates into I.wsi; returns |W|.  Coordinates
-// with x_j != 0 stay in W.  sh: at least kBatchWaves ints of scratch.
-__device__ __forceinline__ int32_t batch_screen(const BatchInstance &I, double lambda,
-                                                double gap_abs, double ginf, int *shi) {
+// forms W from the full refresh's g_j (in I.cs) and gap: copies the records
+// of W into I.ws (coordinate order) and their coordinates into I.wsi;
+// returns |W|.  sh: at least kBatchWaves ints of scratch.
+__device__ __forceinline__ int32_t batch_working_set(const BatchInstance &I, double lambda,
+                                                     double thr, double gap_abs, double ginf,
+                                                     int *shi) {
   const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
   const double s = ginf > lambda ? lambda / ginf : 1.0;
   // a relative margin of 1e-9 on the radius keeps rounding from screening a
@@ -613,7 +621,8 @@ __device__ __forceinline__ int32_t batch_screen(const BatchInstance &I, double l
       r = gld(I.cs + j);
       r.x = gld_agent(&I.cs[j].x);  // workers add to x atomically
       const double nrm = r.L > 0.0 ? sqrt(1.0 / (I.beta * r.L)) : 0.0;
-      keep = r.x != 0.0 || (r.L > 0.0 && s * fabs(r.gref) + nrm * rad >= lambda);
+      const bool safe_zero = s * fabs(r.gref) + nrm * rad < lambda;
+      keep = r.x != 0.0 || (r.L > 0.0 && fabs(r.gref) >= thr && !safe_zero);
     }
     const uint64_t bal = __ballot(keep);
     const int within = __popcll(bal & ((1ull << ln) - 1ull));
@@ -1125,7 +1134,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
         batch_async_updates<T, G, R, false, TAIL, SVM>(J, lambda_of_item, k0, per, lds);
       __syncthreads();  // every worker's updates are in before the refresh
     };
-    // Lasso instances with screening: gap-safe working sets
+    // Lasso instances with screening: working sets (strong rule, gap-safe screening)
     const bool screen = !SVM && a.screening && I.loss == kLasso;
     uint64_t k = 0;      // updates per worker so far (the workers' stream offset)
     uint64_t total = 0;  // updates of the instance so far
@@ -1139,6 +1148,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
       // updates of this lambda, against the cap of max_epochs epochs of n
       const double cap = (double)a.max_epochs * (double)I.n;
       double done = 0.0;
+      // the strong rule's threshold for the step from the previous lambda
+      const double lambda_prev = l > 0 ? a.lambdas[(int64_t)i * a.path_len + l - 1] : lambda;
+      const double thr = fmin(lambda, 2.0 * lambda - lambda_prev);
       int32_t nw = I.n;          // working-set size (n: all coordinates)
       int32_t nw_last = I.n;     // the size of the last set the workers drew from
       double tol_w = a.gap_tol;  // the working set's gap target
@@ -1146,8 +1158,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
         int32_t w_count = 0;
         if (screen) {
           // the full refresh just run holds g = A^T r and the gap
-          w_count = batch_screen(I, lambda, s_res.primal - s_res.dual, s_res.grad_inf,
-                                 reinterpret_cast<int *>(sh));
+          w_count = batch_working_set(I, lambda, thr, s_res.primal - s_res.dual, s_res.grad_inf,
+                                      reinterpret_cast<int *>(sh));
         }
         if (screen && w_count >= 1 && 10 * (int64_t)w_count <= 9 * (int64_t)I.n) {
           // the screened problem: uniform draws from W with W's ESO beta
