@@ -237,7 +237,9 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   const bool ws = set->screening && !svm && set->loss == ACDC_LOSS_L1_SQUARE && b->async;
   CoordRec *dws = nullptr;
   int32_t *dwsi = nullptr;
-  if (ws && ((st = balloc(b, &dws, tot_n)) != ACDC_OK || (st = balloc(b, &dwsi, tot_n)) != ACDC_OK))
+  double *dan = nullptr;  // Anderson snapshots of the working sets
+  if (ws && ((st = balloc(b, &dws, tot_n)) != ACDC_OK || (st = balloc(b, &dwsi, tot_n)) != ACDC_OK ||
+             (st = balloc(b, &dan, tot_n * kAaSnapshots)) != ACDC_OK))
     return bail(st);
   // one pad entry after the last segment: workers load a padding lane's
   // entry unconditionally (from the segment start, which for an empty last
@@ -271,6 +273,7 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     I.cs = b->dcs + on;
     I.ws = ws ? dws + on : nullptr;
     I.wsi = ws ? dwsi + on : nullptr;
+    I.anx = ws ? dan + on * kAaSnapshots : nullptr;
     I.ent = dent + onnz * esz;
     I.rent = drent + onnz;
     I.nnz = (int32_t)p->nnz;

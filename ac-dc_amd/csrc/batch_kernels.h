@@ -64,6 +64,11 @@ struct BatchInstance {
   int32_t loss;          // kLasso, kNnls (x >= 0, no lambda) or kSvm (SVM: the
                          // coordinates are samples, x is alpha, CoordRec.L is
                          // y_j / (beta ||x_j||^2): its sign is the label)
+  int32_t wperm;         // nonzero on a working set: every epoch of the workers
+                         // visits each coordinate once (see batch_async_updates)
+  uint32_t wkey;         // the working set's permutation key (one per formation)
+  double *anx;           // working sets: kAaSnapshots x n snapshots of x for the
+                         // Anderson extrapolation, or null
   double scale;          // NNLS: ||A^T b||_inf, the KKT residual's scale
   double beta;           // ESO beta of the records' L_j = 1/(beta ||a_j||^2)
   uint64_t seed;
@@ -600,38 +605,94 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
 // |<a_j, u*>| = lambda).  The workers draw uniformly from W, with the ESO
 // beta of W: 1 + (omega_W - 1)(tau_W - 1)/(|W| - 1), omega_W the most
 // entries of W in one row, tau_W = min(tau, |W|) workers.
+//
+// W is capped: when the support and the surviving candidates exceed the cap
+// (twice the support, at least kWsMin), only the candidates with the largest
+// |g_j| enter, the prioritised working sets of Celer (Massias, Gramfort and
+// Salmon, ICML 2018).  At small lambda the strong rule keeps a large share
+// of the coordinates whose |g_j| sits just under lambda, while the solution
+// needs few of them; the cap keeps each epoch of W near the support's size.
+// The ranking is a histogram of |g_j| / lambda over [thr / lambda, 1) in
+// kWsBins bins, with every violator (|g_j| >= lambda) in a bin of its own at
+// the top: the candidates of the highest bins that fit the cap enter, and
+// the violators always do.
+constexpr int kWsBins = 128;
+constexpr int32_t kWsMin = 256;
+static_assert(kWsBins + kBatchWaves + 1 <= 2 * kRefreshStats * kBatchWaves,
+              "the histogram and the waves' counts fit the refresh scratch");
 
 // forms W from the full refresh's g_j (in I.cs) and gap: copies the records
 // of W into I.ws (coordinate order) and their coordinates into I.wsi;
-// returns |W|.  sh: at least kBatchWaves ints of scratch.
+// returns |W|.  cap: the largest |W| the candidates may fill (the support
+// always enters whole).  shi: at least kWsBins + kBatchWaves + 1 ints of
+// scratch.
 __device__ __forceinline__ int32_t batch_working_set(const BatchInstance &I, double lambda,
                                                      double thr, double gap_abs, double ginf,
-                                                     int *shi) {
+                                                     int64_t cap, int *shi) {
   const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
+  int *hist = shi, *wc = shi + kWsBins;
   const double s = ginf > lambda ? lambda / ginf : 1.0;
   // a relative margin of 1e-9 on the radius keeps rounding from screening a
   // coordinate on the boundary
   const double rad = sqrt(fmax(2.0 * gap_abs, 0.0)) * (1.0 + 1e-9) + 1e-15 * lambda;
+  const double lo = thr / lambda, span = 1.0 - lo;
+  // -1: not kept; kWsBins: support; else the candidate's bin
+  auto classify = [&](const CoordRec &r) -> int {
+    if (r.x != 0.0) return kWsBins;
+    const double nrm = r.L > 0.0 ? sqrt(1.0 / (I.beta * r.L)) : 0.0;
+    const double ag = fabs(r.gref);
+    if (!(r.L > 0.0 && ag >= thr) || s * ag + nrm * rad < lambda) return -1;
+    const double q = ag / lambda;
+    if (q >= 1.0 || !(span > 0.0)) return kWsBins - 1;
+    return min(kWsBins - 2, max(0, (int)((q - lo) / span * (double)(kWsBins - 1))));
+  };
+  auto load = [&](int32_t j) {
+    CoordRec r = gld(I.cs + j);
+    r.x = gld_agent(&I.cs[j].x);  // workers add to x atomically
+    return r;
+  };
+  // pass 1: the support's size and the candidates' histogram
+  for (int k = tid; k < kWsBins + kBatchWaves + 1; k += kBatchBlock) shi[k] = 0;
+  __syncthreads();
+  int nsup = 0;
+  for (int32_t j = tid; j < I.n; j += kBatchBlock) {
+    const int c = classify(load(j));
+    if (c == kWsBins)
+      ++nsup;
+    else if (c >= 0)
+      atomicAdd(hist + c, 1);
+  }
+  atomicAdd(wc, nsup);
+  __syncthreads();
+  // the lowest bin that still fits: candidates of bins >= cut enter
+  if (tid == 0) {
+    const int64_t room = cap - (int64_t)wc[0];
+    int cut = kWsBins - 1;  // violators always enter
+    int64_t acc = hist[kWsBins - 1];
+    while (cut > 0 && acc + hist[cut - 1] <= room) acc += hist[--cut];
+    wc[1] = cut;
+  }
+  __syncthreads();
+  const int cut = wc[1];
+  __syncthreads();  // wc is reused as the waves' counts below
+  // pass 2: compaction in coordinate order
   int32_t base = 0;
   for (int32_t j0 = 0; j0 < I.n; j0 += kBatchBlock) {
     const int32_t j = j0 + tid;
     bool keep = false;
     CoordRec r{};
     if (j < I.n) {
-      r = gld(I.cs + j);
-      r.x = gld_agent(&I.cs[j].x);  // workers add to x atomically
-      const double nrm = r.L > 0.0 ? sqrt(1.0 / (I.beta * r.L)) : 0.0;
-      const bool safe_zero = s * fabs(r.gref) + nrm * rad < lambda;
-      keep = r.x != 0.0 || (r.L > 0.0 && fabs(r.gref) >= thr && !safe_zero);
+      r = load(j);
+      keep = classify(r) >= cut;
     }
     const uint64_t bal = __ballot(keep);
     const int within = __popcll(bal & ((1ull << ln) - 1ull));
-    if (ln == 0) shi[wv] = __popcll(bal);
+    if (ln == 0) wc[wv] = __popcll(bal);
     __syncthreads();
     int off = 0, tot = 0;
 #pragma unroll
     for (int k = 0; k < kBatchWaves; ++k) {
-      const int c = shi[k];
+      const int c = wc[k];
       off += k < wv ? c : 0;
       tot += c;
     }
@@ -640,7 +701,7 @@ __device__ __forceinline__ int32_t batch_working_set(const BatchInstance &I, dou
       *gp(I.wsi + base + off + within) = j;
     }
     base += tot;
-    __syncthreads();  // shi is rewritten by the next chunk
+    __syncthreads();  // wc is rewritten by the next chunk
   }
   return base;
 }
@@ -785,6 +846,141 @@ __device__ __forceinline__ void batch_ws_writeback(const BatchInstance &I, int32
   for (int32_t u = threadIdx.x; u < nw; u += kBatchBlock)
     *gp(&I.cs[gld(I.wsi + u)].x) = gld_agent(&I.ws[u].x);
   __syncthreads();
+}
+
+// ------------------------------------------------- Anderson extrapolation
+// The workers on a working set repeat one cyclic order (batch_async_updates),
+// so the map F from x at one restricted refresh to x at the next is the same
+// throughout (up to the interleaving of concurrent updates).  Near the
+// solution, where the signs of the support no longer change, F is affine,
+// and Anderson acceleration (Anderson, J. ACM 1965; for coordinate descent,
+// Bertrand and Massias, AISTATS 2021) extrapolates from the last
+// kAaSnapshots iterates X_0 .. X_K: c minimises || sum_i c_i (X_{i+1} - X_i) ||
+// subject to sum_i c_i = 1 (the regularised normal equations of the K x K
+// Gram matrix of the differences), and the candidate sum_i c_i X_{i+1} is
+// kept only when its FP64 objective is below that of X_K.  On the
+// near-interpolating pairs of a lambda path (support close to m, where a
+// coordinate epoch contracts the error very little) this cuts the epochs a
+// lambda needs several times over.
+constexpr int kAaDepth = 5;                 // differences per extrapolation
+constexpr int kAaSnapshots = kAaDepth + 1;  // iterates it uses
+
+// x of the working set's records into snapshot slot h (stride: the
+// instance's n, the slots' size)
+__device__ __forceinline__ void batch_aa_snapshot(const BatchInstance &J, int32_t nw,
+                                                  int64_t stride, int h) {
+  double *X = J.anx + (int64_t)h * stride;
+  for (int32_t u = threadIdx.x; u < nw; u += kBatchBlock) *gp(X + u) = gld_agent(&J.cs[u].x);
+  __syncthreads();
+}
+
+// snapshot slot h back into the working set's records
+__device__ __forceinline__ void batch_aa_restore(const BatchInstance &J, int32_t nw,
+                                                 int64_t stride, int h) {
+  const double *X = J.anx + (int64_t)h * stride;
+  // (snapshots are read at agent scope: a slot's older contents may sit in
+  // this CU's vector cache)
+  for (int32_t u = threadIdx.x; u < nw; u += kBatchBlock) *gp(&J.cs[u].x) = gld_agent(X + u);
+  __syncthreads();
+}
+
+// The extrapolation from the snapshots in slots h0, h0 + 1, ... (mod
+// kAaSnapshots, oldest first) into the working set's records.  Returns
+// false, records untouched, when the system is singular or not finite.
+// sh: at least kAaDepth (kAaDepth + 3) / 2 + 1 doubles of scratch.
+__device__ __forceinline__ bool batch_aa_extrapolate(const BatchInstance &J, int32_t nw,
+                                                     int64_t stride, int h0, double *sh) {
+  constexpr int K = kAaDepth, NP = K * (K + 1) / 2;
+  const int tid = threadIdx.x;
+  __syncthreads();
+  if (tid < NP + K + 1) sh[tid] = 0.0;
+  __syncthreads();
+  double part[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) part[p] = 0.0;
+  for (int32_t u = tid; u < nw; u += kBatchBlock) {
+    double xs[K + 1];
+#pragma unroll
+    for (int i = 0; i <= K; ++i)
+      xs[i] = gld_agent(J.anx + (int64_t)((h0 + i) % kAaSnapshots) * stride + u);
+    double dd[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) dd[i] = xs[i + 1] - xs[i];
+    int p = 0;
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int b = a; b < K; ++b) part[p++] += dd[a] * dd[b];
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const double v = group_sum<64>(part[p]);
+    if ((tid & 63) == 0) atomicAdd(sh + p, v);  // ds_add_f64
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double g[K][K + 1];  // the Gram matrix and the right-hand side of ones
+    int p = 0;
+    double tr = 0.0;
+    for (int a = 0; a < K; ++a)
+      for (int b = a; b < K; ++b) {
+        g[a][b] = g[b][a] = sh[p++];
+        if (a == b) tr += g[a][a];
+      }
+    for (int a = 0; a < K; ++a) {
+      g[a][a] += 1e-10 * tr / K;
+      g[a][K] = 1.0;
+    }
+    bool ok = tr > 0.0;
+    for (int c = 0; c < K && ok; ++c) {  // Gaussian elimination, partial pivoting
+      int piv = c;
+      for (int r = c + 1; r < K; ++r)
+        if (fabs(g[r][c]) > fabs(g[piv][c])) piv = r;
+      if (!(fabs(g[piv][c]) > 0.0)) {
+        ok = false;
+        break;
+      }
+      if (piv != c)
+        for (int k = c; k <= K; ++k) {
+          const double t = g[c][k];
+          g[c][k] = g[piv][k];
+          g[piv][k] = t;
+        }
+      for (int r = c + 1; r < K; ++r) {
+        const double f = g[r][c] / g[c][c];
+        for (int k = c; k <= K; ++k) g[r][k] -= f * g[c][k];
+      }
+    }
+    double z[K], zs = 0.0;
+    for (int r = K - 1; r >= 0 && ok; --r) {
+      double v = g[r][K];
+      for (int k = r + 1; k < K; ++k) v -= g[r][k] * z[k];
+      z[r] = v / g[r][r];
+      zs += z[r];
+    }
+    ok = ok && fabs(zs) > 0.0 && isfinite(zs);
+    for (int i = 0; i < K && ok; ++i) {
+      sh[NP + i] = z[i] / zs;
+      ok = isfinite(sh[NP + i]);
+    }
+    sh[NP + K] = ok ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const bool ok = sh[NP + K] != 0.0;
+  if (ok) {
+    double c[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) c[i] = sh[NP + i];
+    for (int32_t u = tid; u < nw; u += kBatchBlock) {
+      double v = 0.0;
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        v += c[i] * gld_agent(J.anx + (int64_t)((h0 + i + 1) % kAaSnapshots) * stride + u);
+      *gp(&J.cs[u].x) = v;
+    }
+  }
+  __syncthreads();
+  return ok;
 }
 
 // nsteps synchronous steps of one instance; S slots per G-lane group.  The
@@ -979,8 +1175,32 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
     T cv[R];
   };
   AsyncCoordCache<G> coords;
+  // On a working set (I.wperm) the workers share one cyclic order instead:
+  // update q = k tau + w of the instance (k the worker's update count)
+  // takes position q mod n of one keyed permutation of W (the Feistel
+  // permutation of the synchronous steps, keys from Philox at counter
+  // wkey + 2^63, apart from the steps' counters), the same in every epoch.
+  // Each epoch visits every coordinate of W once, the tau updates in flight
+  // are distinct coordinates (the tau-nice sampling of the ESO), and the
+  // epochs repeat one map of x, which the Anderson extrapolation of the
+  // working sets needs.  Updates are requested in increasing k, so the
+  // position advances by tau.
+  const Perm wp = Perm::make((uint32_t)max(I.n, 1));
+  uint32_t wq_pos = 0;
+  StepKeys wkeys{};
+  if (I.wperm) {
+    wq_pos = (uint32_t)((k0 * (uint64_t)I.tau + w) % (uint64_t)I.n);
+    wkeys = step_keys(I.seed, (uint64_t)I.wkey | (1ull << 63));
+  }
   auto record = [&](uint64_t k, Slot &c) {
-    c.j = coords.get(I.seed, w, k0 + k, (uint32_t)I.n);
+    if (I.wperm) {
+      (void)k;
+      c.j = wp(wq_pos, wkeys);
+      wq_pos += (uint32_t)I.tau;
+      if (wq_pos >= (uint32_t)I.n) wq_pos -= (uint32_t)I.n;  // tau <= n: one wrap at most
+    } else {
+      c.j = coords.get(I.seed, w, k0 + k, (uint32_t)I.n);
+    }
     c.r = gld(I.cs + c.j);
   };
   // every lane loads (padding lanes re-read the segment's first entry, or
@@ -1154,12 +1374,14 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
       int32_t nw = I.n;          // working-set size (n: all coordinates)
       int32_t nw_last = I.n;     // the size of the last set the workers drew from
       double tol_w = a.gap_tol;  // the working set's gap target
+      uint32_t wkey = 0;         // working sets formed for this lambda
       while (ok && gap > a.gap_tol && done < cap) {
         int32_t w_count = 0;
         if (screen) {
-          // the full refresh just run holds g = A^T r and the gap
+          // the full refresh just run holds g = A^T r, the gap and the support
+          const int64_t wcap = max((int64_t)kWsMin, 2 * (int64_t)s_res.nnz_x);
           w_count = batch_working_set(I, lambda, thr, s_res.primal - s_res.dual, s_res.grad_inf,
-                                      reinterpret_cast<int *>(sh));
+                                      wcap, reinterpret_cast<int *>(sh));
         }
         if (screen && w_count >= 1 && 10 * (int64_t)w_count <= 9 * (int64_t)I.n) {
           // the screened problem: uniform draws from W with W's ESO beta
@@ -1171,6 +1393,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           J.cs = I.ws;
           J.n = nw;
           J.tau = min(I.tau, nw);
+          J.wperm = 1;
+          J.wkey = wkey++;
           const double bw = nw > 1 ? 1.0 + (double)(om - 1) * (double)(J.tau - 1) / ((double)nw - 1.0)
                                    : 1.0;
           for (int32_t u = threadIdx.x; u < nw; u += kBatchBlock)
@@ -1178,13 +1402,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           __syncthreads();
           const double screen_gap = gap;
           double gap_w = gap;
-          while (done < cap) {
-            const uint64_t per =
-                ((uint64_t)nw * (uint64_t)a.refresh_period + J.tau - 1) / J.tau;
-            updates(J, k, per);
-            k += per;
-            done += (double)per * J.tau;
-            total += per * (uint64_t)J.tau;
+          auto refresh_ws = [&]() {
             if (in_lds ? I.m <= a.lds_doubles : false)
               batch_refresh_ws<T, true, true>(J, nw, lambda, lds, sh, &s_res);
             else if (in_lds)
@@ -1192,18 +1410,51 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
             else
               batch_refresh_ws<T, false, false>(J, nw, lambda, lds, sh, &s_res);
             gap_w = s_res.rel_gap;
-            // back to the whole problem once W's gap meets its target, or
-            // has fallen a hundredfold since the screening (a new screening
-            // then shrinks W), or is not finite (the full refresh below
-            // reports it)
-            if (!finite_eval(s_res) || gap_w <= tol_w || gap_w <= 1e-2 * screen_gap) break;
+          };
+          // back to the whole problem once W's gap meets its target, or has
+          // fallen a hundredfold since the screening (a new screening then
+          // shrinks W), or is not finite (the full refresh below reports it)
+          auto w_done = [&]() {
+            return !finite_eval(s_res) || gap_w <= tol_w || gap_w <= 1e-2 * screen_gap;
+          };
+          int aa_n = 0, aa_h = 0;  // snapshots held, next slot
+          while (done < cap) {
+            const uint64_t per =
+                ((uint64_t)nw * (uint64_t)a.refresh_period + J.tau - 1) / J.tau;
+            updates(J, k, per);
+            k += per;
+            done += (double)per * J.tau;
+            total += per * (uint64_t)J.tau;
+            refresh_ws();
+            if (w_done()) break;
+            if (I.anx) {
+              const int newest = aa_h;
+              batch_aa_snapshot(J, nw, I.n, newest);
+              aa_h = (aa_h + 1) % kAaSnapshots;
+              if (++aa_n == kAaSnapshots) {
+                const double p_last = s_res.primal;
+                if (batch_aa_extrapolate(J, nw, I.n, aa_h, sh)) {
+                  refresh_ws();
+                  if (!(finite_eval(s_res) && s_res.primal < p_last)) {
+                    batch_aa_restore(J, nw, I.n, newest);  // rejected
+                    refresh_ws();
+                  }
+                  if (w_done()) break;
+                }
+                // the next history starts from the current x
+                batch_aa_snapshot(J, nw, I.n, aa_h);
+                aa_h = (aa_h + 1) % kAaSnapshots;
+                aa_n = 1;
+              }
+            }
           }
           batch_ws_writeback(I, nw);
           refresh(lambda);
           gap = s_res.rel_gap;
           ok = finite_eval(s_res);
           // W met its target but the whole problem did not (the dual point
-          // of W is scaled by W's gradients only): aim W lower
+          // of W is scaled by W's gradients only): aim W lower; the
+          // violators outside W, if any, rank first at the next formation
           if (gap > a.gap_tol && gap_w <= tol_w) tol_w *= 0.25;
           nw = I.n;
         } else {
