@@ -408,9 +408,29 @@ __device__ __forceinline__ void st_coherent(T *p, T v) {
 // barrier, so only the residual gather and x_j remain on the critical path.
 constexpr int kPersistentBlock = 1024;  // one block per CU: 256 barrier arrivals
 
+// Sum over aligned groups of G = 128 or 256 lanes of a block (several waves
+// per group): wave sums, then each group adds its waves' sums from LDS in
+// wave order.  Every thread of the block must call it (block barriers).
+template <int G, typename T>
+__device__ __forceinline__ T block_group_sum(T v, T *red) {
+  static_assert(G % 64 == 0 && G > 64, "groups of whole waves");
+  v = group_sum<64>(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int w0 = (int)(threadIdx.x / G) * (G / 64);
+  T t = red[w0];
+#pragma unroll
+  for (int k = 1; k < G / 64; ++k) t += red[w0 + k];
+  __syncthreads();  // red is rewritten by the next call
+  return t;
+}
+
+// G = 8 .. 64 lanes of one wave per slot, or G = kBlock (256): four waves per
+// slot (the SVM's long sample rows), whose partial sums meet in LDS.
 template <typename T, int G, int LOSS, int R>
 __global__ __launch_bounds__(kPersistentBlock) void k_sync_persistent(
     StepArgs<T> a, uint64_t step0, uint64_t nsteps, GridSync *gs, unsigned int spin_limit) {
+  __shared__ T red[kPersistentBlock / 64];
   const int lane = threadIdx.x & (G - 1);
   const int64_t slot = ((int64_t)blockIdx.x * kPersistentBlock + threadIdx.x) / G;
   const bool active = slot < a.tau;  // group-uniform
@@ -433,14 +453,34 @@ __global__ __launch_bounds__(kPersistentBlock) void k_sync_persistent(
   if (active) prefetch(step0);
   for (uint64_t s = 0; s < nsteps; ++s) {
     double c = 0.0;
+    T acc = 0;
     if (active) {
-      T acc = 0;
 #pragma unroll
       for (int r = 0; r < R; ++r)  // padding lanes issue no load
         if (p0 + lane + r * G < p1) acc += cv[r] * ld_coherent(&a.res[ci[r]]);
-      for (int32_t p = p0 + lane + R * G; p < p1; p += G)
-        acc += a.A.val[p] * ld_coherent(&a.res[a.A.idx[p]]);
+      // entries past the cached ones: four per lane in flight (their
+      // indices, then their gathers), as long rows bound the step
+      int32_t p = p0 + lane + R * G;
+      for (; p + 3 * G < p1; p += 4 * G) {
+        int32_t ix[4];
+        T vv[4], g4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          ix[u] = a.A.idx[p + u * G];
+          vv[u] = a.A.val[p + u * G];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) g4[u] = ld_coherent(&a.res[ix[u]]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += vv[u] * g4[u];
+      }
+      for (; p < p1; p += G) acc += a.A.val[p] * ld_coherent(&a.res[a.A.idx[p]]);
+    }
+    if constexpr (G <= 64)
       acc = group_sum<G>(acc);
+    else
+      acc = block_group_sum<G>(acc, red);  // every thread, active or not
+    if (active) {
       const double xj = ld_coherent(&a.x[j]);
       const double d = coord_delta<T, LOSS>(a, j, (double)acc, xj, &c);
       if (lane == 0) st_coherent(&a.x[j], xj + d);
@@ -452,8 +492,19 @@ __global__ __launch_bounds__(kPersistentBlock) void k_sync_persistent(
 #pragma unroll
         for (int r = 0; r < R; ++r)
           if (p0 + lane + r * G < p1) unsafeAtomicAdd(&a.res[ci[r]], ct * cv[r]);
-        for (int32_t p = p0 + lane + R * G; p < p1; p += G)
-          unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
+        int32_t p = p0 + lane + R * G;
+        for (; p + 3 * G < p1; p += 4 * G) {  // four entries per lane per round trip
+          int32_t ix[4];
+          T vv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            ix[u] = a.A.idx[p + u * G];
+            vv[u] = a.A.val[p + u * G];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) unsafeAtomicAdd(&a.res[ix[u]], ct * vv[u]);
+        }
+        for (; p < p1; p += G) unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
       }
       if (s + 1 < nsteps) prefetch(step0 + s + 1);
     }

@@ -315,7 +315,7 @@ int auto_step_lanes(int64_t nnz, int64_t nseg) {
 }
 
 // ---- dispatch: value type x lanes x loss (WIDE: also a block per coordinate,
-// for the per-step kernels only)
+// for the synchronous per-step and persistent kernels)
 template <template <typename, int, int> class F, bool WIDE = false, typename... Args>
 void dispatch3(const acdc_solver *s, int lanes, Args &&...args) {
   auto go = [&](auto tag_t, auto tag_g) {
@@ -682,7 +682,7 @@ acdc_status run_persistent(acdc_solver *s, uint64_t step0, uint64_t nsteps) {
     const uint64_t chunk = std::min(kStepsPerLaunch, nsteps - done);
     HIP_TRY(hipMemsetAsync(s->gsync, 0, sizeof(GridSync), s->stream));
     hipError_t err = hipSuccess;
-    dispatch3<PersistentKernel>(s, s->lanes, step0 + done, chunk, &err);
+    dispatch3<PersistentKernel, true>(s, s->lanes, step0 + done, chunk, &err);
     if (err != hipSuccess) {
       if (s->set.launch_mode != ACDC_LAUNCH_AUTO)
         return fail(ACDC_ERR_HIP, std::string("cooperative launch: ") + hipGetErrorString(err));
@@ -734,11 +734,10 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
       set->lanes_per_coord != 32 && set->lanes_per_coord != 64 && set->lanes_per_coord != kBlock)
     return fail(ACDC_ERR_INVALID,
                 "acdc_solver_create: lanes_per_coord must be 0, 8, 16, 32, 64 or 256");
-  if (set->lanes_per_coord == kBlock &&
-      (set->step_mode == ACDC_STEP_ASYNC || set->launch_mode == ACDC_LAUNCH_PERSISTENT))
+  if (set->lanes_per_coord == kBlock && set->step_mode == ACDC_STEP_ASYNC)
     return fail(ACDC_ERR_INVALID,
                 "acdc_solver_create: 256 lanes per coordinate (a block per coordinate) is for "
-                "synchronous steps with per-step launches");
+                "synchronous steps");
   if (!(set->lambda >= 0) || (loss == kSvm && !(set->lambda > 0)))
     return fail(ACDC_ERR_INVALID, "acdc_solver_create: lambda must be >= 0 (> 0 for the SVM)");
   if (set->refresh_period < 0)
@@ -850,20 +849,24 @@ extern "C" acdc_status acdc_solver_create(const acdc_problem *p, int32_t loss,
   s->async = set->step_mode == ACDC_STEP_ASYNC;
   for (size_t c = 0; c + 1 < p->cptr.size(); ++c)
     s->maxlen = std::max<int64_t>(s->maxlen, (int64_t)p->cptr[c + 1] - p->cptr[c]);
+  // (a block per coordinate takes the persistent grid only when asked: on
+  // the SVM's sample rows a step waits for its longest row either way, and
+  // per-step launches measured 5% faster)
   if (!s->async && set->tau >= 2 &&
-      (set->launch_mode == ACDC_LAUNCH_AUTO || set->launch_mode == ACDC_LAUNCH_PERSISTENT) &&
-      s->lanes != kBlock) {
+      ((set->launch_mode == ACDC_LAUNCH_AUTO && s->lanes != kBlock) ||
+       set->launch_mode == ACDC_LAUNCH_PERSISTENT)) {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t prop;
     (void)hipGetDevice(&dev);
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
-      dispatch3<Occupancy>(s, s->lanes, &per_cu);
+      dispatch3<Occupancy, true>(s, s->lanes, &per_cu);
       // at most one 1024-thread block per CU: a 512-block grid at two blocks
       // per CU (which the occupancy query allows) did not make every block
       // arrive at the grid barrier within its bound on the MI355X
       s->coresident_blocks = (int64_t)std::min(per_cu, 1) * prop.multiProcessorCount;
       // with automatic lanes, narrow the groups until tau slots fit the grid
-      if (!set->lanes_per_coord)
+      // (a block per coordinate keeps its width: per-step launches then)
+      if (!set->lanes_per_coord && s->lanes <= 64)
         while (s->lanes > 8 && div_up(set->tau * s->lanes, kPersistentBlock) > s->coresident_blocks)
           s->lanes /= 2;
     }

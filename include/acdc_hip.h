@@ -84,8 +84,8 @@ typedef struct acdc_settings {
   int32_t step_mode;       /* acdc_step_mode */
   int32_t lanes_per_coord; /* 0 = auto, else 8, 16, 32 or 64 lanes per
                               sampled coordinate, or 256 (a whole block per
-                              coordinate: synchronous steps with per-step
-                              launches, for long segments; auto picks it
+                              coordinate: synchronous steps, per-step or
+                              persistent, for long segments; auto picks it
                               when segments average over 192 entries) */
   int32_t refresh_period;  /* FP32 path: FP64 residual refresh every this
                               many epochs of ceil(n/tau) steps inside

@@ -57,13 +57,10 @@ def test_solver_argument_errors(acdc):
         acdc.Solver(p, acdc.LOSS_SVM_DUAL_HINGE, lam=0.1)
     with pytest.raises(acdc.AcdcError, match="lanes"):
         acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=0.1, lanes_per_coord=12)
-    # a block per coordinate: synchronous steps with per-step launches only
+    # a block per coordinate: synchronous steps only
     with pytest.raises(acdc.AcdcError, match="block per coordinate"):
         acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=0.1, lanes_per_coord=256,
                     step_mode=acdc.STEP_ASYNC)
-    with pytest.raises(acdc.AcdcError, match="block per coordinate"):
-        acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=0.1, tau=2, lanes_per_coord=256,
-                    launch_mode=acdc.LAUNCH_PERSISTENT)
     # the streamed scatter: synchronous steps, tau >= 2, n coefficients in LDS
     with pytest.raises(acdc.AcdcError, match="streamed scatter"):
         acdc.Solver(p, acdc.LOSS_L1_SQUARE, lam=0.1, tau=1,

@@ -86,7 +86,7 @@ def test_tau1_trajectory_fp64(acdc, orc, c1):
 
 
 @pytest.mark.parametrize("lanes,launch", [(8, 1), (16, 1), (32, 1), (64, 1), (256, 1),
-                                          (8, 2), (16, 2), (32, 2), (64, 2), (16, 3), (256, 3)])
+                                          (8, 2), (16, 2), (32, 2), (64, 2), (256, 2), (16, 3), (256, 3)])
 def test_sync_tau_fp64_matches_oracle(acdc, orc, c1, lanes, launch):
     """launch 1: one gather + one scatter launch per step; 2: persistent grid;
     3: per step with the streamed scatter over the transposed copy; 256
