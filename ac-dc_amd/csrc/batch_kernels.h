@@ -64,9 +64,9 @@ struct BatchInstance {
   int32_t loss;          // kLasso, kNnls (x >= 0, no lambda) or kSvm (SVM: the
                          // coordinates are samples, x is alpha, CoordRec.L is
                          // y_j / (beta ||x_j||^2): its sign is the label)
-  int32_t wperm;         // nonzero on a working set: every epoch of the workers
-                         // visits each coordinate once (see batch_async_updates)
-  uint32_t wkey;         // the working set's permutation key (one per formation)
+  int32_t wperm;         // nonzero on a working set, whose records are stored in
+                         // a keyed random order: the workers sweep them
+                         // cyclically (see batch_async_updates)
   double *anx;           // working sets: kAaSnapshots x n snapshots of x for the
                          // Anderson extrapolation, or null
   double scale;          // NNLS: ||A^T b||_inf, the KKT residual's scale
@@ -625,10 +625,12 @@ static_assert(kWsBins + kBatchWaves + 1 <= 2 * kRefreshStats * kBatchWaves,
 // of W into I.ws (coordinate order) and their coordinates into I.wsi;
 // returns |W|.  cap: the largest |W| the candidates may fill (the support
 // always enters whole).  shi: at least kWsBins + kBatchWaves + 1 ints of
-// scratch.
+// scratch.  The records are then put in the order of the keyed permutation
+// of [0, |W|) for counter key + 2^63 (the Feistel permutation of the
+// synchronous steps, apart from their counters), through I.anx as scratch.
 __device__ __forceinline__ int32_t batch_working_set(const BatchInstance &I, double lambda,
                                                      double thr, double gap_abs, double ginf,
-                                                     int64_t cap, int *shi) {
+                                                     int64_t cap, uint64_t key, int *shi) {
   const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
   int *hist = shi, *wc = shi + kWsBins;
   const double s = ginf > lambda ? lambda / ginf : 1.0;
@@ -702,6 +704,26 @@ __device__ __forceinline__ int32_t batch_working_set(const BatchInstance &I, dou
     }
     base += tot;
     __syncthreads();  // wc is rewritten by the next chunk
+  }
+  // pass 3: slot u takes the record of slot perm(u) (through scratch: 32
+  // bytes of record and 4 of index per slot, within the kAaSnapshots x n
+  // doubles of I.anx)
+  if (base > 1) {
+    const Perm wp = Perm::make((uint32_t)base);
+    const StepKeys wk = step_keys(I.seed, key | (1ull << 63));
+    CoordRec *trec = reinterpret_cast<CoordRec *>(I.anx);
+    int32_t *tidx = reinterpret_cast<int32_t *>(I.anx + 4 * (int64_t)I.n);
+    for (int32_t u = tid; u < base; u += kBatchBlock) {
+      const uint32_t v = wp((uint32_t)u, wk);
+      trec[u] = I.ws[v];
+      *gp(tidx + u) = gld(I.wsi + v);
+    }
+    __syncthreads();
+    for (int32_t u = tid; u < base; u += kBatchBlock) {
+      I.ws[u] = trec[u];
+      *gp(I.wsi + u) = gld(tidx + u);
+    }
+    __syncthreads();
   }
   return base;
 }
@@ -850,8 +872,9 @@ __device__ __forceinline__ void batch_ws_writeback(const BatchInstance &I, int32
 
 // ------------------------------------------------- Anderson extrapolation
 // The workers on a working set repeat one cyclic order (batch_async_updates),
-// so the map F from x at one restricted refresh to x at the next is the same
-// throughout (up to the interleaving of concurrent updates).  Near the
+// so the map F from x at one snapshot to x at the next (refresh_period epochs
+// of W) is the same throughout (up to the interleaving of concurrent
+// updates).  Near the
 // solution, where the signs of the support no longer change, F is affine,
 // and Anderson acceleration (Anderson, J. ACM 1965; for coordinate descent,
 // Bertrand and Massias, AISTATS 2021) extrapolates from the last
@@ -864,6 +887,7 @@ __device__ __forceinline__ void batch_ws_writeback(const BatchInstance &I, int32
 // lambda needs several times over.
 constexpr int kAaDepth = 5;                 // differences per extrapolation
 constexpr int kAaSnapshots = kAaDepth + 1;  // iterates it uses
+constexpr int kWsRefreshPeriods = 4;        // snapshot periods per restricted refresh
 
 // x of the working set's records into snapshot slot h (stride: the
 // instance's n, the slots' size)
@@ -1177,25 +1201,19 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
   AsyncCoordCache<G> coords;
   // On a working set (I.wperm) the workers share one cyclic order instead:
   // update q = k tau + w of the instance (k the worker's update count)
-  // takes position q mod n of one keyed permutation of W (the Feistel
-  // permutation of the synchronous steps, keys from Philox at counter
-  // wkey + 2^63, apart from the steps' counters), the same in every epoch.
-  // Each epoch visits every coordinate of W once, the tau updates in flight
-  // are distinct coordinates (the tau-nice sampling of the ESO), and the
-  // epochs repeat one map of x, which the Anderson extrapolation of the
-  // working sets needs.  Updates are requested in increasing k, so the
-  // position advances by tau.
-  const Perm wp = Perm::make((uint32_t)max(I.n, 1));
+  // takes slot q mod n of W, whose records batch_working_set stored in a
+  // keyed random order, the same in every epoch.  Each epoch visits every
+  // coordinate of W once, the tau updates in flight are distinct
+  // coordinates (the tau-nice sampling of the ESO), and the epochs repeat
+  // one map of x, which the Anderson extrapolation of the working sets
+  // needs.  Updates are requested in increasing k, so the slot advances by
+  // tau.
   uint32_t wq_pos = 0;
-  StepKeys wkeys{};
-  if (I.wperm) {
-    wq_pos = (uint32_t)((k0 * (uint64_t)I.tau + w) % (uint64_t)I.n);
-    wkeys = step_keys(I.seed, (uint64_t)I.wkey | (1ull << 63));
-  }
+  if (I.wperm) wq_pos = (uint32_t)((k0 * (uint64_t)I.tau + w) % (uint64_t)I.n);
   auto record = [&](uint64_t k, Slot &c) {
     if (I.wperm) {
       (void)k;
-      c.j = wp(wq_pos, wkeys);
+      c.j = wq_pos;
       wq_pos += (uint32_t)I.tau;
       if (wq_pos >= (uint32_t)I.n) wq_pos -= (uint32_t)I.n;  // tau <= n: one wrap at most
     } else {
@@ -1381,7 +1399,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           // the full refresh just run holds g = A^T r, the gap and the support
           const int64_t wcap = max((int64_t)kWsMin, 2 * (int64_t)s_res.nnz_x);
           w_count = batch_working_set(I, lambda, thr, s_res.primal - s_res.dual, s_res.grad_inf,
-                                      wcap, reinterpret_cast<int *>(sh));
+                                      wcap, ((uint64_t)l << 32) | wkey++,
+                                      reinterpret_cast<int *>(sh));
         }
         if (screen && w_count >= 1 && 10 * (int64_t)w_count <= 9 * (int64_t)I.n) {
           // the screened problem: uniform draws from W with W's ESO beta
@@ -1394,7 +1413,6 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           J.n = nw;
           J.tau = min(I.tau, nw);
           J.wperm = 1;
-          J.wkey = wkey++;
           const double bw = nw > 1 ? 1.0 + (double)(om - 1) * (double)(J.tau - 1) / ((double)nw - 1.0)
                                    : 1.0;
           for (int32_t u = threadIdx.x; u < nw; u += kBatchBlock)
@@ -1417,7 +1435,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           auto w_done = [&]() {
             return !finite_eval(s_res) || gap_w <= tol_w || gap_w <= 1e-2 * screen_gap;
           };
+          // snapshots of x every refresh_period epochs of W; restricted
+          // refreshes (gap check, FP64 anchor) every kWsRefreshPeriods of
+          // them and around each extrapolation
           int aa_n = 0, aa_h = 0;  // snapshots held, next slot
+          int since = 0;           // periods since the last restricted refresh
           while (done < cap) {
             const uint64_t per =
                 ((uint64_t)nw * (uint64_t)a.refresh_period + J.tau - 1) / J.tau;
@@ -1425,8 +1447,12 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
             k += per;
             done += (double)per * J.tau;
             total += per * (uint64_t)J.tau;
-            refresh_ws();
-            if (w_done()) break;
+            const bool aa_due = I.anx && aa_n + 1 == kAaSnapshots;
+            if (!I.anx || aa_due || ++since >= kWsRefreshPeriods) {
+              refresh_ws();
+              since = 0;
+              if (w_done()) break;
+            }
             if (I.anx) {
               const int newest = aa_h;
               batch_aa_snapshot(J, nw, I.n, newest);
@@ -1435,6 +1461,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
                 const double p_last = s_res.primal;
                 if (batch_aa_extrapolate(J, nw, I.n, aa_h, sh)) {
                   refresh_ws();
+                  since = 0;
                   if (!(finite_eval(s_res) && s_res.primal < p_last)) {
                     batch_aa_restore(J, nw, I.n, newest);  // rejected
                     refresh_ws();

@@ -629,7 +629,7 @@ def main():
     ap.add_argument("--probe-device", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-path", type=int, default=32)
-    ap.add_argument("--batch-max-epochs", type=int, default=400,
+    ap.add_argument("--batch-max-epochs", type=int, default=500,
                     help="cap on the updates of one (instance, lambda) pair, in epochs of n")
     args = ap.parse_args()
     if args.gpus != 1:
