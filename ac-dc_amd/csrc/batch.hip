@@ -124,6 +124,7 @@ acdc_status enqueue_run(acdc_batch *b) {
   a.results = b->dres;
   a.lds_doubles = (int32_t)(kLdsBytes / 8);
   a.screening = b->set.screening;
+  a.spectral = b->set.beta_mode == ACDC_BETA_SPECTRAL;
   void *args[] = {&a};
   const size_t shmem = (kLdsBytes / tsz) * tsz;
   BHIP(hipLaunchKernel(batch_kernel(b), dim3(std::min(b->blocks, b->count)), dim3(kBatchBlock),

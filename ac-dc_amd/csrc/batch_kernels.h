@@ -152,6 +152,7 @@ struct BatchArgs {
   BatchResult *results;  // instances x path_len
   int32_t lds_doubles;   // FP64 rows the dynamic LDS holds (refresh residual)
   int32_t screening;     // Lasso, asynchronous workers: gap-safe working sets
+  int32_t spectral;      // working sets take the spectral ESO constant of W
 };
 
 // the results of lambdas l .. path_len - 1 of an abandoned instance
@@ -771,6 +772,91 @@ __device__ __forceinline__ int32_t batch_ws_omega(const BatchInstance &I, int32_
   dr_view<T, LDSR>(I, lds).zero(I.m);
   __syncthreads();
   return om;
+}
+
+// The spectral ESO constant of a working set, lambda_max of
+// D^-1/2 A_W^T A_W D^-1/2 (D the squared column norms), by kWsPowerSteps
+// steps of the power method (the rule of mSDCA.cpp:188-265, on the columns
+// of W): y = sum_u a_u v_u / ||a_u|| into the correction residual's storage,
+// then v'_u = <a_u, y> / ||a_u||, with v and v' in I.anx.  Returns the last
+// Rayleigh estimate; zeroes the storage again.  Column norms come from the
+// records (L_u = 1 / (beta ||a_u||^2) at formation).
+constexpr int kWsPowerSteps = 12;
+
+template <typename T, bool LDSR>
+__device__ __forceinline__ double batch_ws_lmax(const BatchInstance &I, int32_t nw, T *lds,
+                                                double *sh) {
+  const int tid = threadIdx.x;
+  const int32_t split = LDSR ? I.m : I.lds_rows;
+  T *yl = lds;
+  T *yg = static_cast<T *>(I.drg);
+  double *v = I.anx, *vn = I.anx + I.n;
+  auto inv_norm = [&](int32_t u) {
+    const double L = gld(&I.ws[u].L);
+    return L > 0.0 ? sqrt(I.beta * L) : 0.0;
+  };
+  // a start with no symmetry to miss the top eigenvector by: 1 + a hash of u
+  for (int32_t u = tid; u < nw; u += kBatchBlock)
+    *gp(v + u) = 1.0 + (double)(mix32((uint32_t)u, 0x9E3779B9u) >> 8) * 0x1p-24;
+  __syncthreads();
+  double est = 1.0;
+  constexpr int G = 16;  // lanes per column
+  const int grp = tid / G, lane = tid & (G - 1);
+  for (int it = 0; it < kWsPowerSteps; ++it) {
+    dr_view<T, LDSR>(I, lds).zero(I.m);
+    __syncthreads();
+    for (int32_t u = grp; u < nw; u += kBatchBlock / G) {
+      const int32_t p0 = gld(&I.ws[u].p0), len = gld(&I.ws[u].len);
+      const double c = gld_agent(v + u) * inv_norm(u);
+      for (int32_t q = lane; q < len; q += G) {
+        const RefEntry e = gld(I.rent + p0 + q);
+        const T t = (T)(c * e.val);
+        if (e.row < split)
+          atomicAdd(yl + e.row, t);
+        else
+          __hip_atomic_fetch_add(gp(yg + (e.row - split)), t, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    __syncthreads();
+    double sq = 0.0, vv = 0.0;
+    for (int32_t u = grp; u < nw; u += kBatchBlock / G) {
+      const int32_t p0 = gld(&I.ws[u].p0), len = gld(&I.ws[u].len);
+      double acc = 0.0;
+      for (int32_t q = lane; q < len; q += G) {
+        const RefEntry e = gld(I.rent + p0 + q);
+        acc += e.val * (double)(e.row < split ? yl[e.row] : gld_agent(yg + (e.row - split)));
+      }
+      acc = group_sum<G>(acc) * inv_norm(u);
+      if (lane == 0) {
+        const double x = gld_agent(v + u);
+        *gp(vn + u) = acc;
+        sq += acc * acc;
+        vv += x * x;
+      }
+    }
+    sq = group_sum<64>(sq);
+    vv = group_sum<64>(vv);
+    if ((tid & 63) == 0) {
+      sh[tid >> 6] = sq;
+      sh[kBatchWaves + (tid >> 6)] = vv;
+    }
+    __syncthreads();
+    double tsq = 0.0, tvv = 0.0;
+    for (int k = 0; k < kBatchWaves; ++k) {
+      tsq += sh[k];
+      tvv += sh[kBatchWaves + k];
+    }
+    __syncthreads();
+    if (!(tsq > 0.0) || !(tvv > 0.0)) break;
+    est = sqrt(tsq / tvv);  // ||A v|| / ||v|| for the Gram operator, v' = G v
+    const double s = 1.0 / sqrt(tsq);
+    for (int32_t u = tid; u < nw; u += kBatchBlock) *gp(v + u) = gld_agent(vn + u) * s;
+    __syncthreads();
+  }
+  dr_view<T, LDSR>(I, lds).zero(I.m);
+  __syncthreads();
+  return est;
 }
 
 // FP64 refresh restricted to the working set: r64 = A x - b from W's nonzero
@@ -1413,7 +1499,16 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           J.n = nw;
           J.tau = min(I.tau, nw);
           J.wperm = 1;
-          const double bw = nw > 1 ? 1.0 + (double)(om - 1) * (double)(J.tau - 1) / ((double)nw - 1.0)
+          // the ESO constant of W: omega_W, or with the spectral rule the
+          // power method's estimate on W (10% margin: it approaches
+          // lambda_max from below), whichever is smaller
+          double sigma = (double)om;
+          if (a.spectral && I.anx && nw > 1) {
+            const double lm = in_lds ? batch_ws_lmax<T, true>(I, nw, lds, sh)
+                                     : batch_ws_lmax<T, false>(I, nw, lds, sh);
+            sigma = fmin(sigma, fmax(1.0, 1.1 * lm));
+          }
+          const double bw = nw > 1 ? 1.0 + (sigma - 1.0) * (double)(J.tau - 1) / ((double)nw - 1.0)
                                    : 1.0;
           for (int32_t u = threadIdx.x; u < nw; u += kBatchBlock)
             *gp(&I.ws[u].L) = gld(&I.ws[u].L) * (I.beta / bw);
