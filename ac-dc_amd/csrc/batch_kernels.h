@@ -1488,8 +1488,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
                                       wcap, ((uint64_t)l << 32) | wkey++,
                                       reinterpret_cast<int *>(sh));
         }
-        if (screen && w_count >= 1 && 10 * (int64_t)w_count <= 9 * (int64_t)I.n) {
-          // the screened problem: uniform draws from W with W's ESO beta
+        if (screen && w_count >= 1) {
+          // the working-set problem (W may hold every coordinate: the cyclic
+          // sweep and the extrapolation still apply), with W's ESO beta
           nw = w_count;
           nw_last = nw;
           const int32_t om = in_lds ? batch_ws_omega<T, true>(I, nw, lds, reinterpret_cast<int *>(sh))
