@@ -418,21 +418,26 @@ def test_batch_spectral_beta_svm(acdc, svm_batch):
     b.close()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
-def test_batch_screening_reaches_oracle_gap(acdc, orc, small_batch, prec):
-    """Gap-safe working sets (screening on) for asynchronous Lasso instances
-    along a path down to 1e-3 lambda_max: every pair converges, the oracle's
-    evaluation of each instance's last x has relative gap <= 1e-6 and equals
-    the device's report, and the workers drew from working sets smaller than
-    n along the path."""
+@pytest.mark.parametrize("prec,beta", [("fp32", "omega"), ("fp64", "omega"),
+                                       ("fp32", "spectral")])
+def test_batch_screening_reaches_oracle_gap(acdc, orc, small_batch, prec, beta):
+    """Working sets (screening on: strong rule, gap-safe screening, a cap by
+    |g_j| rank, a cyclic sweep, Anderson extrapolation; under the spectral
+    rule W's ESO constant from the power method) for asynchronous Lasso
+    instances along a path down to 1e-3 lambda_max: every pair converges,
+    the oracle's evaluation of each instance's last x has relative gap
+    <= 1e-6 and equals the device's report, and the workers drew from
+    working sets smaller than n along the path."""
     ds = small_batch
     lams = lambdas_of(acdc, ds, 8, 1e-3)
     probs = [d.problem() for d in ds]
     precision = acdc.PREC_FP32_REFRESH if prec == "fp32" else acdc.PREC_FP64
+    beta_mode = acdc.BETA_SPECTRAL if beta == "spectral" else acdc.BETA_OMEGA
     # the serial method needs ~2,600-6,700 epochs of n at the last lambda of
     # the three wide instances (measured with the oracle on like data)
     b = acdc.Batch(probs, lams, seed=11, precision=precision, refresh_period=5, max_epochs=40000,
-                   gap_tol=1e-6, use_graph=True, step_mode=acdc.STEP_ASYNC, screening=True)
+                   gap_tol=1e-6, use_graph=True, step_mode=acdc.STEP_ASYNC, screening=True,
+                   beta_mode=beta_mode)
     b.run()
     res = b.results()
     print("epochs", res["epochs"].tolist(), "ws", res["ws_size"].tolist(), flush=True)
