@@ -132,15 +132,23 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(StepArgs<T> a, uint64_t 
   if (slot >= a.tau) return;  // whole groups leave together
   const uint32_t j = a.perm((uint32_t)slot, step_keys(a.seed, step));
   const int32_t p0 = a.A.ptr[j], p1 = a.A.ptr[j + 1];
-  // four entries per lane in flight: their indices, then their gathers
+  // the coordinate's x, L, anchor and label, loaded with the segment bounds
+  // rather than after the dot (one dependent round trip less per step)
+  const double xj = a.x[j], Lj = a.L[j];
+  const double gj = a.gref ? a.gref[j] : 0.0;
+  const double yj = LOSS == kSvm ? a.y[j] : 0.0;
+  // U entries per lane in flight: their indices, then their gathers (eight
+  // for a block per coordinate: the step waits for its longest row, whose
+  // rounds of dependent loads this halves)
+  constexpr int U = G == kBlock ? 8 : 4;
   T acc = 0;
   int32_t p = p0 + lane;
-  for (; p + 3 * G < p1; p += 4 * G) {
-    int32_t r[4];
+  for (; p + (U - 1) * G < p1; p += U * G) {
+    int32_t r[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) r[u] = a.A.idx[p + u * G];
+    for (int u = 0; u < U; ++u) r[u] = a.A.idx[p + u * G];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc += a.A.val[p + u * G] * a.res[r[u]];
+    for (int u = 0; u < U; ++u) acc += a.A.val[p + u * G] * a.res[r[u]];
   }
   for (; p < p1; p += G) acc += a.A.val[p] * a.res[a.A.idx[p]];
   if constexpr (G == kBlock) {  // a block per coordinate: the waves meet in LDS
@@ -154,9 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(StepArgs<T> a, uint64_t 
   } else {
     acc = group_sum<G>(acc);
   }
-  const double xj = a.x[j];
   double c;
-  const double d = coord_delta<T, LOSS>(a, j, (double)acc, xj, &c);
+  const double d = prox_delta<LOSS>(gj + (double)acc, xj, Lj, yj, a.sc, &c);
   if (lane == 0) {
     if (a.cvec) {
       // this slot's coordinate of the previous step leaves the other half
@@ -188,17 +195,18 @@ __global__ __launch_bounds__(kBlock) void k_step_scatter(StepArgs<T> a) {
   const int32_t j = a.coord[slot];
   const T ct = (T)c;
   const int32_t p0 = a.A.ptr[j], p1 = a.A.ptr[j + 1];
+  constexpr int U = G == kBlock ? 8 : 4;  // entries per lane per round trip
   int32_t p = p0 + lane;
-  for (; p + 3 * G < p1; p += 4 * G) {
-    int32_t r[4];
-    T v[4];
+  for (; p + (U - 1) * G < p1; p += U * G) {
+    int32_t r[U];
+    T v[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       r[u] = a.A.idx[p + u * G];
       v[u] = a.A.val[p + u * G];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) unsafeAtomicAdd(&a.res[r[u]], ct * v[u]);
+    for (int u = 0; u < U; ++u) unsafeAtomicAdd(&a.res[r[u]], ct * v[u]);
   }
   for (; p < p1; p += G) unsafeAtomicAdd(&a.res[a.A.idx[p]], ct * a.A.val[p]);
 }
