@@ -10,6 +10,11 @@
 //   pointers as text, <prefix>_rowid_<rank><suffix> the row indices as raw
 //   L, <prefix>_values_<rank><suffix> the values as raw D (host byte order);
 //   b from a text file.  Short or missing files are errors.
+// Dense CSV (readCSVFile, parse_data_size_from_CSV_file and
+//   parse_data_from_CSV_file, cpp/src/utils/file_reader.h:138-194): m lines of
+//   n comma-separated numbers, n read from the first line; stored
+//   column-major (entry (row, col) at row + col m).  Unlike the reference, a
+//   short or malformed line is an error.
 // LIBSVM rows (loadDistributedSparseSVMRowData,
 //   cpp/src/utils/distributed_instances_loader.h:68-193): "label idx:val ..."
 //   per sample, 1-based feature indices shifted to 0-based, label 0 read as -1;
@@ -126,6 +131,67 @@ void loadLibSVMData(const std::string &file, ProblemData<L, D> &part) {
   }
   part.n = (L)part.b.size();
   part.m = (L)maxFeature;
+}
+
+// Dense CSV matrix into column-major `data` (m rows, n columns).
+template <typename F>
+void readCSVFile(std::vector<F> &data, unsigned int &ld, unsigned int &m, unsigned int &n,
+                 const std::string &file) {
+  std::ifstream f(file.c_str());
+  if (!f.is_open()) throw std::runtime_error("file not found: " + file);
+  std::vector<std::vector<F>> rows;
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty() || line == "\r") continue;
+    std::vector<F> row;
+    std::stringstream ls(line);
+    std::string cell;
+    while (std::getline(ls, cell, ',')) {
+      std::stringstream cs(cell);
+      double v = 0;
+      if (!(cs >> v)) throw std::runtime_error("malformed number '" + cell + "' in " + file);
+      row.push_back((F)v);
+    }
+    if (!rows.empty() && row.size() != rows[0].size())
+      throw std::runtime_error("rows of different lengths in " + file);
+    rows.push_back(std::move(row));
+  }
+  m = (unsigned int)rows.size();
+  n = m ? (unsigned int)rows[0].size() : 0u;
+  ld = m;
+  data.assign((size_t)m * n, (F)0);
+  for (unsigned int r = 0; r < m; ++r)
+    for (unsigned int c = 0; c < n; ++c) data[r + (size_t)c * m] = rows[r][c];
+}
+
+// A dense CSV matrix as a CSC instance (its nonzeros) and a CSV b, as the
+// reference's lambda-path experiment loads them (lasso_console_solver.cpp:76-112).
+template <typename L, typename D>
+void loadDenseCSV(const std::string &matrixFile, const std::string &bFile,
+                  ProblemData<L, D> &instance) {
+  std::vector<D> a;
+  unsigned int ld = 0, m = 0, n = 0;
+  readCSVFile(a, ld, m, n, matrixFile);
+  instance.m = (L)m;
+  instance.n = (L)n;
+  instance.A_csc_col_ptr.assign(1, 0);
+  instance.A_csc_row_idx.clear();
+  instance.A_csc_values.clear();
+  for (unsigned int c = 0; c < n; ++c) {
+    for (unsigned int r = 0; r < m; ++r) {
+      const D v = a[r + (size_t)c * m];
+      if (v != 0) {
+        instance.A_csc_row_idx.push_back((L)r);
+        instance.A_csc_values.push_back(v);
+      }
+    }
+    instance.A_csc_col_ptr.push_back((L)instance.A_csc_row_idx.size());
+  }
+  std::vector<D> b;
+  unsigned int bm = 0, bn = 0;
+  readCSVFile(b, ld, bm, bn, bFile);
+  if (bm != m || bn != 1) throw std::runtime_error("b must be an m x 1 CSV matching A: " + bFile);
+  instance.b = b;
 }
 
 }  // namespace InputOuputHelper

@@ -137,3 +137,60 @@ def test_executor_beta_rules(acdc, tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip().endswith("ok")
+
+
+PATH_CLI = os.path.join(ROOT, "ac-dc_amd", "bin", "acdc_hip_lasso_path")
+
+
+def dense_csv(path_a, path_b, d):
+    """The dataset as the dense CSV files the lambda-path experiment reads."""
+    a = np.zeros((d.m, d.n))
+    for j in range(d.n):
+        a[d.idx[d.ptr[j]:d.ptr[j + 1]], j] += d.vals[d.ptr[j]:d.ptr[j + 1]]
+    np.savetxt(path_a, a, fmt="%.17g", delimiter=",")
+    np.savetxt(path_b, d.rhs[:, None], fmt="%.17g", delimiter=",")
+    return a
+
+
+def test_lasso_path_cli_reads_dense_csv(acdc, tmp_path):
+    """The CSV reader (rows x columns from the file, column-major storage) and
+    the argument checks; without a device the run stops after loading."""
+    assert os.path.exists(PATH_CLI), "build the front end first (make -C ac-dc_amd)"
+    assert subprocess.run([PATH_CLI], capture_output=True).returncode == 2
+    d = acdc.gen_lasso(7, 5, nnz_fixed=3, seed=2)
+    dense_csv(tmp_path / "a.csv", tmp_path / "b.csv", d)
+    r = subprocess.run([PATH_CLI, "-A", str(tmp_path / "a.csv"), "-b", str(tmp_path / "b.csv"),
+                        "-o", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert "Loaded data with m=7 and n=5" in r.stdout
+    (tmp_path / "bad.csv").write_text("1,2\n3\n")
+    r = subprocess.run([PATH_CLI, "-A", str(tmp_path / "bad.csv"), "-b", str(tmp_path / "b.csv")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "different lengths" in r.stderr
+
+
+@pytest.mark.gpu
+def test_lasso_path_cli_matches_oracle(acdc, orc, tmp_path):
+    """The lambda path of lasso_console_solver.cpp on the device: the log's
+    rows and the per-lambda x files; the last lambda's x is the oracle's
+    optimum (relative gap <= 1e-6 evaluated by the FP64 oracle at the
+    printed lambda, which carries six significant digits as the reference
+    prints it)."""
+    d = acdc.gen_lasso(150, 400, nnz_fixed=8, xstar_nnz=8, noise=0.01, lambda_frac=0.1, seed=23)
+    dense_csv(tmp_path / "a.csv", tmp_path / "b.csv", d)
+    hi = float(d.lambda_max)
+    r = subprocess.run([PATH_CLI, "-A", str(tmp_path / "a.csv"), "-b", str(tmp_path / "b.csv"),
+                        "-H", repr(hi), "-L", repr(hi * 0.05), "-N", "4", "-R", "40", "-m", "50",
+                        "-t", "4", "-p", "0", "-g", "1e-12", "-o", str(tmp_path)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lams = [float(v) for v in re.findall(r"^lambda (\S+)$", r.stdout, re.M)]
+    assert len(lams) == 5
+    rows = open(tmp_path / "lasso.log").read().strip().splitlines()
+    assert rows[0].split(",")[:3] == ["4", str(d.n), str(d.m)]
+    assert all(len(row.split(",")) == 8 for row in rows[1:])
+    files = sorted(p.name for p in tmp_path.glob("lasso_x_*.log"))
+    assert len(files) == 5
+    name = re.findall(r"^lambda (\S+)$", r.stdout, re.M)[-1]
+    x = np.loadtxt(tmp_path / f"lasso_x_{name}.log")
+    e = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[-1]).eval(x)
+    assert e["rel_gap"] <= 1e-6, e
