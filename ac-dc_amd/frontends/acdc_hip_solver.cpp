@@ -11,6 +11,9 @@
 //   -c <n>            number of bulks (the reference stores -c into a bool)
 //   -i <0|1>          show initial/intermediate/last objective values
 //   -r <file>         CSV log file
+//   -T <seconds>      stop once the bulks' pure computation time reaches it
+//                     (the reference parses -T and ignores it; here it sets
+//                     settings.total_execution_time)
 // and the HIP knobs (no reference counterpart):
 //   -s <seed>  -p <0 fp64 | 1 fp32+refresh>  -R <refresh epochs>  -L <lanes>
 //   -P <0 auto | 1 per-step launches | 2 persistent grid | 3 per step, streamed scatter>
@@ -39,6 +42,7 @@
 static void usage() {
   std::cerr << "usage: acdc_hip_solver -A <data> [-b <b file>] -l <lambda> [-t tau] [-f loss]\n"
                "                       [-I updates per bulk] [-c bulks] [-i 0|1] [-r log]\n"
+               "                       [-T seconds of computation]\n"
                "                       [-s seed] [-p precision] [-R refresh epochs] [-L lanes]\n"
                "                       [-P launch mode] [-M step mode: 0 sync, 1 async]\n"
                "                       [-B beta rule: -1 reference, 0 omega, 2 none, 3 spectral]\n"
@@ -51,7 +55,7 @@ int main(int argc, char *argv[]) {
   std::string matrixA, vectorB, logPath;
   double lambda = 0;
   int c;
-  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:s:p:R:L:P:M:B:S:h")) != -1) {
+  while ((c = getopt(argc, argv, "A:b:l:t:f:I:c:i:r:T:s:p:R:L:P:M:B:S:h")) != -1) {
     switch (c) {
       case 'A': matrixA = optarg; break;
       case 'b': vectorB = optarg; break;
@@ -68,6 +72,7 @@ int main(int argc, char *argv[]) {
         break;
       }
       case 'r': logPath = optarg; break;
+      case 'T': settings.total_execution_time = std::atof(optarg); break;
       case 's': settings.hipSeed = std::strtoull(optarg, nullptr, 10); break;
       case 'p': settings.hipPrecision = std::atoi(optarg); break;
       case 'R': settings.hipRefreshPeriod = std::atoi(optarg); break;

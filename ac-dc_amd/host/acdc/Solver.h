@@ -2,6 +2,9 @@
 // iters_bulkIterations_count bulks of innerIterations updates, accumulate the
 // pure computation time around each bulk, optionally refresh the residuals,
 // and log one CSV row per bulk.  Works with any AbstractEngineExecutor.
+// With total_execution_time > 0 the loop also stops once the pure
+// computation time reaches it (the time budget of the reference's GPU
+// solver loop, randomL2L1Solver.h:168).
 #pragma once
 #include <iomanip>
 #include <iostream>
@@ -54,6 +57,9 @@ class Solver {
         objVal = executor.getObjectiveValue();
         if (settings->verbose) log(executor.getResultLogRow(statistics));
       }
+      if (settings->total_execution_time > 0 &&
+          statistics.elapsedPureComputationTime >= settings->total_execution_time)
+        break;
     }
     if (settings->showLastObjectiveValue) {
       objVal = executor.getObjectiveValue();

@@ -63,6 +63,10 @@ class OptimizationSettings {
   double forcedSigma = -1;  // > 0: use this ESO beta
   unsigned long iters_bulkIterations_count = 10;
   unsigned long innerIterations = 100;
+  // seconds of pure computation after which runSolver stops early (0: the
+  // bulk count alone); the reference's field (settingsAndStatistics.h:35),
+  // which its GPU solver loops on (randomL2L1Solver.h:168)
+  double total_execution_time = 0;
 
   // HIP path (no counterpart in the reference)
   unsigned long long hipSeed = 1;             // Philox key of the coordinate stream

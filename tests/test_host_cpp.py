@@ -194,3 +194,23 @@ def test_lasso_path_cli_matches_oracle(acdc, orc, tmp_path):
     x = np.loadtxt(tmp_path / f"lasso_x_{name}.log")
     e = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[-1]).eval(x)
     assert e["rel_gap"] <= 1e-6, e
+
+
+@pytest.mark.gpu
+def test_cli_time_budget_stops_the_bulks(acdc, tmp_path):
+    """-T: the bulk loop ends once its pure computation time reaches the
+    budget, long before the (huge) bulk count."""
+    import time
+    d = acdc.gen_lasso(300, 1200, nnz_fixed=10, xstar_nnz=12, noise=0.01, lambda_frac=0.1,
+                       seed=19)
+    prefix = str(tmp_path / "lasso")
+    write_csc(prefix, d)
+    t0 = time.perf_counter()
+    r = subprocess.run([CLI, "-A", prefix, "-b", prefix + "_b.txt", "-l", repr(d.lam),
+                        "-t", "16", "-f", "0", "-I", str(d.n), "-c", "100000000", "-i", "1",
+                        "-T", "0.2", "-r", str(tmp_path / "log.csv")],
+                       capture_output=True, text=True, timeout=120)
+    wall = time.perf_counter() - t0
+    assert r.returncode == 0, r.stderr
+    rows = open(tmp_path / "log.csv").read().strip().splitlines()
+    assert 1 <= len(rows) < 100000000 and wall < 60
