@@ -108,14 +108,20 @@ class BatchSettings(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("step_mode", ctypes.c_int32),
                 ("tau", ctypes.c_int32), ("queue_chunk", ctypes.c_int32),
                 ("loss", ctypes.c_int32), ("beta_mode", ctypes.c_int32),
-                ("screening", ctypes.c_int32)]
+                ("screening", ctypes.c_int32), ("ws_newton_iters", ctypes.c_int32)]
+
+
+# CG iterations of a working set's support Newton step when screening is on
+# and the caller names none
+WS_NEWTON_ITERS = 200
 
 
 class _BatchResult(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ("lambda_", "primal", "dual", "rel_gap", "nnz_x")] + \
                [("epochs", ctypes.c_int32), ("converged", ctypes.c_int32),
                 ("updates", ctypes.c_uint64), ("device_us", ctypes.c_double),
-                ("ws_size", ctypes.c_double), ("status", ctypes.c_int32)]
+                ("ws_size", ctypes.c_double), ("status", ctypes.c_int32),
+                ("newton_steps", ctypes.c_int32)]
 
 
 def _load() -> ctypes.CDLL:
@@ -438,6 +444,12 @@ class Solver:
             self.close()
 
 
+def _newton_iters(screening, iters):
+    if iters is None:
+        return WS_NEWTON_ITERS if screening else 0
+    return int(iters)
+
+
 class Batch:
     """Many Lasso, NNLS (loss=LOSS_NNLS) or hinge-SVM dual
     (loss=LOSS_SVM_DUAL_HINGE, problems from Problem.csr) instances, each along
@@ -447,7 +459,8 @@ class Batch:
     def __init__(self, problems, lambdas, *, seed=1, precision=PREC_FP32_REFRESH,
                  lanes_per_coord=0, refresh_period=5, max_epochs=200, gap_tol=1e-6,
                  use_graph=True, step_mode=STEP_SYNC, tau=0, queue_chunk=0,
-                 loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA, screening=False):
+                 loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA, screening=False,
+                 ws_newton_iters=None):
         lam = _f64(lambdas)
         if lam.ndim != 2 or lam.shape[0] != len(problems):
             raise ValueError("lambdas must be (instances, path_len)")
@@ -457,7 +470,7 @@ class Batch:
         arr = (_P * self.count)(*[p._h for p in problems])
         st = BatchSettings(seed, precision, lanes_per_coord, refresh_period, max_epochs, gap_tol,
                            int(bool(use_graph)), step_mode, tau, queue_chunk, loss, beta_mode,
-                           int(bool(screening)))
+                           int(bool(screening)), _newton_iters(screening, ws_newton_iters))
         h = _P()
         _check(lib.acdc_batch_create(arr, self.count, _ptr(lam, ctypes.c_double), self.path_len,
                                      ctypes.byref(st), ctypes.byref(h)))
@@ -476,7 +489,7 @@ class Batch:
         dt = np.dtype([("lambda", "f8"), ("primal", "f8"), ("dual", "f8"), ("rel_gap", "f8"),
                        ("nnz_x", "f8"), ("epochs", "i4"), ("converged", "i4"), ("updates", "u8"),
                        ("device_us", "f8"), ("ws_size", "f8"), ("status", "i4"),
-                       ("_pad", "i4")])
+                       ("newton_steps", "i4")])
         return np.frombuffer(bytes(out), dtype=dt).reshape(self.count, self.path_len).copy()
 
     def x(self, i: int) -> np.ndarray:
@@ -502,12 +515,13 @@ class Batch:
 
 def batch_tune(problems, *, seed=1, precision=PREC_FP32_REFRESH, step_mode=STEP_ASYNC,
                refresh_period=5, max_epochs=200, gap_tol=1e-6, use_graph=True,
-               loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA, screening=False) -> dict:
+               loss=LOSS_L1_SQUARE, beta_mode=BETA_OMEGA, screening=False,
+               ws_newton_iters=None) -> dict:
     """acdc_batch_tune: lanes per coordinate, tau and queue chunking for a batch."""
     arr = (_P * len(problems))(*[p._h for p in problems])
     base = BatchSettings(seed, precision, 0, refresh_period, max_epochs, gap_tol,
                          int(bool(use_graph)), step_mode, 0, 0, loss, beta_mode,
-                         int(bool(screening)))
+                         int(bool(screening)), _newton_iters(screening, ws_newton_iters))
     out = BatchSettings()
     score = ctypes.c_double()
     _check(lib.acdc_batch_tune(arr, len(problems), ctypes.byref(base), ctypes.byref(out),

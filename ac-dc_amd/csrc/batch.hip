@@ -52,6 +52,7 @@ struct acdc_batch {
   bool fp32 = true;
   bool async = false;  // ACDC_STEP_ASYNC inside instances
   bool svm = false;    // hinge-SVM dual instances (kernels of their own)
+  bool ws = false;     // screened Lasso instances with asynchronous workers (kernels of their own)
   int lanes = 16;
   int32_t count = 0, path_len = 0;
   std::vector<int64_t> n, m, xoff;
@@ -99,8 +100,9 @@ void destroy_batch(acdc_batch *b) {
 }
 
 const void *batch_kernel(const acdc_batch *b) {
-  return b->svm ? batch_svm_kernel(b->fp32, b->lanes, b->async, b->maxlen)
-                : batch_kernel_for<false>(b->fp32, b->lanes, b->async, b->maxlen);
+  if (b->svm) return batch_svm_kernel(b->fp32, b->lanes, b->async, b->maxlen);
+  if (b->ws) return batch_ws_kernel(b->fp32, b->lanes, b->maxlen);
+  return batch_kernel_for<false>(b->fp32, b->lanes, b->async, b->maxlen);
 }
 
 // enqueue the reset and the persistent launch on b->stream
@@ -125,6 +127,7 @@ acdc_status enqueue_run(acdc_batch *b) {
   a.lds_doubles = (int32_t)(kLdsBytes / 8);
   a.screening = b->set.screening;
   a.spectral = b->set.beta_mode == ACDC_BETA_SPECTRAL;
+  a.ws_newton = b->set.ws_newton_iters;
   void *args[] = {&a};
   const size_t shmem = (kLdsBytes / tsz) * tsz;
   BHIP(hipLaunchKernel(batch_kernel(b), dim3(std::min(b->blocks, b->count)), dim3(kBatchBlock),
@@ -162,6 +165,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: beta_mode must be omega or spectral");
   if (set->screening != 0 && set->screening != 1)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: screening must be 0 or 1");
+  if (set->ws_newton_iters < 0 || set->ws_newton_iters > ACDC_WS_NEWTON_MAX_ITERS)
+    return bfail(ACDC_ERR_INVALID, "acdc_batch_create: ws_newton_iters must be in [0, 10000]");
   if (set->loss != ACDC_LOSS_L1_SQUARE && set->loss != ACDC_LOSS_NNLS &&
       set->loss != ACDC_LOSS_SVM_DUAL_HINGE)
     return bfail(ACDC_ERR_INVALID, "acdc_batch_create: loss must be L1-square, NNLS or SVM dual");
@@ -236,11 +241,12 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   if ((st = balloc(b, &b->dcs, tot_n)) != ACDC_OK) return bail(st);
   // working sets (screened Lasso instances with asynchronous workers only)
   const bool ws = set->screening && !svm && set->loss == ACDC_LOSS_L1_SQUARE && b->async;
+  b->ws = ws;
   CoordRec *dws = nullptr;
   int32_t *dwsi = nullptr;
-  double *dan = nullptr;  // Anderson snapshots of the working sets
+  double *dan = nullptr;  // Anderson snapshots and Newton-step vectors of the working sets
   if (ws && ((st = balloc(b, &dws, tot_n)) != ACDC_OK || (st = balloc(b, &dwsi, tot_n)) != ACDC_OK ||
-             (st = balloc(b, &dan, tot_n * kAaSnapshots)) != ACDC_OK))
+             (st = balloc(b, &dan, tot_n * kWsScratchSlots)) != ACDC_OK))
     return bail(st);
   // one pad entry after the last segment: workers load a padding lane's
   // entry unconditionally (from the segment start, which for an empty last
@@ -274,17 +280,23 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     I.cs = b->dcs + on;
     I.ws = ws ? dws + on : nullptr;
     I.wsi = ws ? dwsi + on : nullptr;
-    I.anx = ws ? dan + on * kAaSnapshots : nullptr;
+    I.anx = ws ? dan + on * kWsScratchSlots : nullptr;
     I.ent = dent + onnz * esz;
     I.rent = drent + onnz;
     I.nnz = (int32_t)p->nnz;
     I.loss = set->loss;  // the LossId values are the ABI's
     I.scale = 0;
-    for (int64_t j = 0; !svm && j < p->ncoord; ++j) {  // ||A^T b||_inf
-      double g = 0;
-      for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) g += p->cval[q] * p->rhs[p->cidx[q]];
+    I.colsq = 0;
+    for (int64_t j = 0; !svm && j < p->ncoord; ++j) {  // ||A^T b||_inf and mean ||a_j||^2
+      double g = 0, c = 0;
+      for (int32_t q = p->cptr[j]; q < p->cptr[j + 1]; ++q) {
+        g += p->cval[q] * p->rhs[p->cidx[q]];
+        c += p->cval[q] * p->cval[q];
+      }
       I.scale = std::max(I.scale, std::fabs(g));
+      I.colsq += c;
     }
+    if (p->ncoord > 0) I.colsq /= (double)p->ncoord;
     I.b = svm ? nullptr : db + om;
     I.r64 = dr64 + om;
     // residual rows beyond the LDS capacity go to the instance's global part
@@ -451,6 +463,7 @@ extern "C" acdc_status acdc_batch_results(acdc_batch *b, acdc_batch_result *out,
     out[k].device_us = (double)r[k].ticks * 1e3 / b->clock_khz;
     out[k].ws_size = r[k].ws_size;
     out[k].status = r[k].status;
+    out[k].newton_steps = r[k].newton;
   }
   return ACDC_OK;
 }

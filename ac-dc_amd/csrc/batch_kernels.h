@@ -67,9 +67,12 @@ struct BatchInstance {
   int32_t wperm;         // nonzero on a working set, whose records are stored in
                          // a keyed random order: the workers sweep them
                          // cyclically (see batch_async_updates)
-  double *anx;           // working sets: kAaSnapshots x n snapshots of x for the
-                         // Anderson extrapolation, or null
+  double *anx;           // working sets: kWsScratchSlots x n doubles: the
+                         // kAaSnapshots snapshots of x of the Anderson
+                         // extrapolation, then the vectors of the support
+                         // Newton steps; or null
   double scale;          // NNLS: ||A^T b||_inf, the KKT residual's scale
+  double colsq;          // Lasso and NNLS: the mean ||a_j||^2 of the columns
   double beta;           // ESO beta of the records' L_j = 1/(beta ||a_j||^2)
   uint64_t seed;
 };
@@ -129,6 +132,7 @@ struct BatchResult {
   double grad_inf;   // max_j |<a_j, r>| of the evaluation
   double ws_size;    // coordinates in the working set at the end (n without screening)
   int32_t status;    // 0, or kBatchNonFinite
+  int32_t newton;    // support Newton steps kept for this lambda
 };
 
 // An evaluation with a NaN or an infinity (in the data or the iterate)
@@ -153,6 +157,7 @@ struct BatchArgs {
   int32_t lds_doubles;   // FP64 rows the dynamic LDS holds (refresh residual)
   int32_t screening;     // Lasso, asynchronous workers: gap-safe working sets
   int32_t spectral;      // working sets take the spectral ESO constant of W
+  int32_t ws_newton;     // working sets: CG iterations of each support Newton step (0: none)
 };
 
 // the results of lambdas l .. path_len - 1 of an abandoned instance
@@ -173,6 +178,7 @@ __device__ __forceinline__ void batch_fail_rest(const BatchArgs &a, int i, int l
     o->grad_inf = last.grad_inf;
     o->ws_size = 0.0;
     o->status = kBatchNonFinite;
+    o->newton = 0;
   }
 }
 
@@ -1093,6 +1099,255 @@ __device__ __forceinline__ bool batch_aa_extrapolate(const BatchInstance &J, int
   return ok;
 }
 
+// ------------------------------------------------------ support Newton steps
+// Near interpolation (support close to m, or above it on the way) the
+// least-squares problem on the support S is ill-conditioned: a coordinate
+// epoch contracts the error by about 1 - 1/kappa of H = A_S^T A_S, so a pair
+// needs tens of thousands of epochs, where conjugate gradients need about
+// sqrt(kappa) products with H.  At a restricted refresh (r64 = A x - b exact,
+// g_u = <a_u, r64> in the records), with s the signs of x on S, the Lasso
+// objective on the orthant of s is the quadratic
+//   1/2 ||A_S x_S - b||^2 + lambda s^T x_S,
+// whose Newton direction d solves H d = -(g_S + lambda s).  When |S| > m
+// (the iterate's support passes m on the way to a solution with at most m
+// nonzeros) H is singular and the right-hand side has a part in its null
+// space, along which CG steps grow without bound; the damped system
+// (H + mu I) d = -(g_S + lambda s), mu = kNewtonDamping times the mean
+// ||a_j||^2 of the instance's columns (the Levenberg-Marquardt damping),
+// bounds them.  At most
+// `iters` CG iterations from d = 0 approximate d.  The projected search then
+// takes the first t in
+// 1, 1/2, ..., 1/128 for which x' = P_s(x_S + t d) -- every coordinate that
+// would change sign set to 0, the orthant projection of OWL-QN (Andrew and
+// Gao, ICML 2007) -- has an FP64 objective below the refresh's primal p0.
+// The workers then continue from x' (coordinates enter and leave S) and the
+// next restricted refresh steps again.  FP64 throughout: q = A_S p lives in
+// the storage of r64 (LDS when its m doubles fit, the instance's global r64
+// otherwise); S is compacted into the six slots of I.anx after the
+// snapshots (x_S, d, the CG residual, p, and 16 bytes of column per
+// coordinate).  Returns whether x' was written to the records; the
+// correction residual dr is zero on return either way.
+constexpr int kWsScratchSlots = kAaSnapshots + 6;  // doubles of I.anx per coordinate
+constexpr int kNewtonSearch = 8;                   // step lengths the search tries
+constexpr double kNewtonDamping = 1e-4;            // mu over the mean ||a_j||^2
+
+// the block's sum of one double per thread, in a fixed order, returned to
+// every thread (so that all of them take the same branches on it)
+__device__ __forceinline__ double block_sum_all(double v, double *sh) {
+  v = group_sum<64>(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int k = 0; k < kBatchWaves; ++k) t += sh[k];
+  __syncthreads();
+  return t;
+}
+
+// a support column of a Newton step: its slot in W and its entries
+struct alignas(16) NewtonCol {
+  int32_t u, p0, len, pad;
+};
+
+
+template <typename T, bool LDSR, bool RL>
+__device__ __forceinline__ bool batch_ws_newton(const BatchInstance &J, int32_t nw, int64_t stride,
+                                             double lambda, double p0, int iters, T *lds,
+                                             double *sh) {
+  const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
+  double *const q = RL ? reinterpret_cast<double *>(lds) : J.r64;
+  double *const xs = J.anx + (int64_t)kAaSnapshots * stride;  // x on S, compacted
+  double *const dv = xs + stride;                             // d
+  double *const rv = dv + stride;                             // CG residual
+  double *const pv = rv + stride;                             // CG direction
+  NewtonCol *const cols = reinterpret_cast<NewtonCol *>(pv + stride);
+  auto qget = [&](int32_t k) -> double { return RL ? q[k] : gld_agent(q + k); };
+  auto qset = [&](int32_t k, double v) {
+    if (RL)
+      q[k] = v;
+    else
+      *gp(q + k) = v;
+  };
+  auto qadd = [&](int32_t k, double v) {
+    if (RL)
+      atomicAdd(q + k, v);  // ds_add_f64
+    else
+      gadd_block(q + k, v);
+  };
+  // S compacted in W order, with res = p = -(g_S + lambda s) and d = 0
+  int *wc = reinterpret_cast<int *>(sh);
+  int32_t ns = 0;
+  double rr = 0.0;
+  for (int32_t u0 = 0; u0 < nw; u0 += kBatchBlock) {
+    const int32_t u = u0 + tid;
+    const double x = u < nw ? gld_agent(&J.cs[u].x) : 0.0;  // workers add to x atomically
+    const bool keep = x != 0.0;
+    const uint64_t bal = __ballot(keep);
+    if (ln == 0) wc[wv] = __popcll(bal);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBatchWaves; ++k) {
+      const int c = wc[k];
+      off += k < wv ? c : 0;
+      tot += c;
+    }
+    if (keep) {
+      const int32_t k = ns + off + __popcll(bal & ((1ull << ln) - 1ull));
+      const CoordRec r = gld(J.cs + u);
+      const double g = -(r.gref + lambda * sign_of(x));
+      *gp(xs + k) = x;
+      *gp(dv + k) = 0.0;
+      *gp(rv + k) = g;
+      *gp(pv + k) = g;
+      cols[k] = NewtonCol{u, r.p0, r.len, 0};
+      rr += g * g;
+    }
+    ns += tot;
+    __syncthreads();  // wc is rewritten by the next chunk
+  }
+  rr = block_sum_all(rr, sh);  // its barriers also publish the stores above
+  const double mu = kNewtonDamping * J.colsq;
+  // A_S passes: each G-lane group takes U columns at a time, their records
+  // and first G entries loaded before any is used (a column's dependent
+  // loads would otherwise be the pass's whole cost)
+  constexpr int G = 16, NG = kBatchBlock / G, U = 4;
+  const int grp = tid / G, lane = tid & (G - 1);
+  // q += c_k a_k, c_k = coef(k) (0: skipped); returns the lane's share of
+  // sum_k c_k^2 ||a_k||^2
+  auto scatter = [&](auto coef) {
+    double sq = 0.0;
+    for (int32_t k0 = grp; k0 < ns; k0 += NG * U) {
+      NewtonCol cl[U];
+      double c[U];
+      RefEntry e[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int32_t k = k0 + j * NG;
+        cl[j] = k < ns ? gld(cols + k) : NewtonCol{0, 0, 0, 0};
+        c[j] = k < ns ? coef(k) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) e[j] = lane < cl[j].len ? gld(J.rent + cl[j].p0 + lane) : RefEntry{};
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (c[j] == 0.0) continue;  // uniform in the group
+        if (lane < cl[j].len) {
+          const double v = e[j].val * c[j];
+          qadd(e[j].row, v);
+          sq += v * v;
+        }
+        for (int32_t k = lane + G; k < cl[j].len; k += G) {
+          const RefEntry f = gld(J.rent + cl[j].p0 + k);
+          const double v = f.val * c[j];
+          qadd(f.row, v);
+          sq += v * v;
+        }
+      }
+    }
+    return sq;
+  };
+  double p2 = rr;  // ||p||^2
+  const double rr0 = rr;
+  int it = 0;
+  while (it < iters && rr > 1e-24 * rr0) {
+    for (int32_t k = tid; k < J.m; k += kBatchBlock) qset(k, 0.0);
+    __syncthreads();
+    // pp = sum_k p_k^2 ||a_k||^2, the curvature's scale
+    double pp = scatter([&](int32_t k) { return gld_agent(pv + k); });
+    __syncthreads();
+    double qq = 0.0;
+    for (int32_t k = tid; k < J.m; k += kBatchBlock) {
+      const double v = qget(k);
+      qq += v * v;
+    }
+    qq = block_sum_all(qq, sh);
+    pp = block_sum_all(pp, sh);
+    const double curv = qq + mu * p2;  // p^T (H + mu I) p
+    if (!(curv > 1e-10 * pp)) break;
+    const double al = rr / curv;
+    // h_k = <a_k, q> + mu p_k: d += al p, res -= al h
+    double rn = 0.0;
+    for (int32_t k0 = grp; k0 < ns; k0 += NG * U) {
+      NewtonCol cl[U];
+      RefEntry e[U];
+      double h[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int32_t k = k0 + j * NG;
+        cl[j] = k < ns ? gld(cols + k) : NewtonCol{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) e[j] = lane < cl[j].len ? gld(J.rent + cl[j].p0 + lane) : RefEntry{};
+#pragma unroll
+      for (int j = 0; j < U; ++j) h[j] = lane < cl[j].len ? e[j].val * qget(e[j].row) : 0.0;
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        for (int32_t k = lane + G; k < cl[j].len; k += G) {
+          const RefEntry f = gld(J.rent + cl[j].p0 + k);
+          h[j] += f.val * qget(f.row);
+        }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int32_t k = k0 + j * NG;
+        const double hs = group_sum<G>(h[j]);
+        if (lane == 0 && k < ns) {
+          const double pk = gld_agent(pv + k);
+          *gp(dv + k) = gld_agent(dv + k) + al * pk;
+          const double r = gld_agent(rv + k) - al * (hs + mu * pk);
+          *gp(rv + k) = r;
+          rn += r * r;
+        }
+      }
+    }
+    rn = block_sum_all(rn, sh);
+    const double be = rn / rr;
+    double pn2 = 0.0;
+    for (int32_t k = tid; k < ns; k += kBatchBlock) {
+      const double pk = gld_agent(rv + k) + be * gld_agent(pv + k);
+      *gp(pv + k) = pk;
+      pn2 += pk * pk;
+    }
+    p2 = block_sum_all(pn2, sh);  // (its barriers publish p)
+    rr = rn;
+    ++it;
+  }
+  // the projected point of column k for step t
+  auto moved = [&](int32_t k, double t) {
+    const double x = gld_agent(xs + k);
+    const double xn = x + t * gld_agent(dv + k);
+    return sign_of(xn) == sign_of(x) ? xn : 0.0;
+  };
+  bool acc = false;
+  for (int s = 0; it > 0 && s < kNewtonSearch && !acc; ++s) {
+    const double t = ldexp(1.0, -s);
+    // q = A x' - b
+    for (int32_t r = tid; r < J.m; r += kBatchBlock) qset(r, -gld(J.b + r));
+    __syncthreads();
+    (void)scatter([&](int32_t k) { return moved(k, t); });
+    double x1 = 0.0;
+    for (int32_t k = tid; k < ns; k += kBatchBlock) x1 += fabs(moved(k, t));
+    __syncthreads();
+    double r2 = 0.0;
+    for (int32_t r = tid; r < J.m; r += kBatchBlock) {
+      const double v = qget(r);
+      r2 += v * v;
+    }
+    r2 = block_sum_all(r2, sh);
+    x1 = block_sum_all(x1, sh);
+    // below p0 by more than the rounding of two differently ordered sums
+    if (0.5 * r2 + lambda * x1 < p0 * (1.0 - 1e-14)) {
+      for (int32_t k = tid; k < ns; k += kBatchBlock)
+        *gp(&J.cs[gld(&cols[k].u)].x) = moved(k, t);
+      acc = true;
+    }
+  }
+  __syncthreads();
+  dr_view<T, LDSR>(J, lds).zero(J.m);
+  __syncthreads();
+  return acc;
+}
+
 // nsteps synchronous steps of one instance; S slots per G-lane group.  The
 // sampled coordinates of a wave's S * (64/G) slots are computed by its first
 // lanes in one pass and broadcast; each slot's record and column are loaded
@@ -1417,7 +1672,10 @@ __device__ __forceinline__ void batch_async_updates(const BatchInstance &I, doub
   if (k < count) update(k, s0, s1, ahead);
 }
 
-template <typename T, int G, int R, bool TAIL, bool SVM>
+// WS: the working-set machinery (screening, Anderson extrapolation, support
+// Newton steps) is compiled in; the kernels without it (NNLS, SVM and
+// unscreened Lasso batches) keep their registers for the workers' loop
+template <typename T, int G, int R, bool TAIL, bool SVM, bool WS>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
@@ -1459,7 +1717,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
       __syncthreads();  // every worker's updates are in before the refresh
     };
     // Lasso instances with screening: working sets (strong rule, gap-safe screening)
-    const bool screen = !SVM && a.screening && I.loss == kLasso;
+    const bool screen = WS && !SVM && a.screening && I.loss == kLasso;
     uint64_t k = 0;      // updates per worker so far (the workers' stream offset)
     uint64_t total = 0;  // updates of the instance so far
     for (int l = 0; l < a.path_len; ++l) {
@@ -1479,6 +1737,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
       int32_t nw_last = I.n;     // the size of the last set the workers drew from
       double tol_w = a.gap_tol;  // the working set's gap target
       uint32_t wkey = 0;         // working sets formed for this lambda
+      if (threadIdx.x == 0) s_res.newton = 0;  // support Newton steps kept (counted in LDS)
       while (ok && gap > a.gap_tol && done < cap) {
         int32_t w_count = 0;
         if (screen) {
@@ -1516,6 +1775,15 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           __syncthreads();
           const double screen_gap = gap;
           double gap_w = gap;
+          // a support Newton step from the restricted refresh just run
+          auto newton = [&](const BatchInstance &Jw, int32_t n_w) {
+            const double p0 = s_res.primal;
+            if (in_lds ? I.m <= a.lds_doubles : false)
+              return batch_ws_newton<T, true, true>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, sh);
+            if (in_lds)
+              return batch_ws_newton<T, true, false>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, sh);
+            return batch_ws_newton<T, false, false>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, sh);
+          };
           auto refresh_ws = [&]() {
             if (in_lds ? I.m <= a.lds_doubles : false)
               batch_refresh_ws<T, true, true>(J, nw, lambda, lds, sh, &s_res);
@@ -1548,6 +1816,12 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
               refresh_ws();
               since = 0;
               if (w_done()) break;
+              if (I.anx && a.ws_newton > 0 && newton(J, nw)) {
+                if (threadIdx.x == 0) ++s_res.newton;
+                refresh_ws();
+                if (w_done()) break;
+                aa_n = 0;  // the snapshots no longer lead to x: a new history
+              }
             }
             if (I.anx) {
               const int newest = aa_h;
@@ -1612,6 +1886,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
         o->grad_inf = s_res.grad_inf;
         o->ws_size = (double)nw_last;
         o->status = 0;
+        o->newton = s_res.newton;
       }
       __syncthreads();
     }
@@ -1636,29 +1911,31 @@ constexpr int async_regs_for(int lanes) {
   return lanes == 64 ? 2 : (lanes == 32 || sizeof(T) == 8 ? 4 : 8);
 }
 
-template <typename T, int G, bool SVM>
+template <typename T, int G, bool SVM, bool WS>
 const void *batch_fn(bool async, bool tail) {
   constexpr int RA = async_regs_for<T>(G);
-  if (async)
-    return tail ? reinterpret_cast<const void *>(&k_batch_async<T, G, RA, true, SVM>)
-                : reinterpret_cast<const void *>(&k_batch_async<T, G, RA, false, SVM>);
+  if (WS || async)  // working sets are for asynchronous workers only
+    return tail ? reinterpret_cast<const void *>(&k_batch_async<T, G, RA, true, SVM, WS>)
+                : reinterpret_cast<const void *>(&k_batch_async<T, G, RA, false, SVM, WS>);
   return reinterpret_cast<const void *>(&k_batch_lasso<T, G, kSlotsPerGroup, regs_for(G), SVM>);
 }
 
 // the batch kernel for a precision, lanes per coordinate (8, 16, 32 or 64),
-// step mode and longest column; SVM = true is instantiated in a translation
-// unit of its own and reached through batch_svm_kernel
-template <bool SVM>
+// step mode and longest column; SVM = true and WS = true (screened Lasso,
+// asynchronous) are each instantiated in a translation unit of their own and
+// reached through batch_svm_kernel and batch_ws_kernel
+template <bool SVM, bool WS = false>
 const void *batch_kernel_for(bool fp32, int lanes, bool async, int64_t maxlen) {
   const int ra = fp32 ? async_regs_for<float>(lanes) : async_regs_for<double>(lanes);
   const bool tail = maxlen > (int64_t)ra * lanes;
   switch (lanes) {
-    case 8: return fp32 ? batch_fn<float, 8, SVM>(async, tail) : batch_fn<double, 8, SVM>(async, tail);
-    case 16: return fp32 ? batch_fn<float, 16, SVM>(async, tail) : batch_fn<double, 16, SVM>(async, tail);
-    case 32: return fp32 ? batch_fn<float, 32, SVM>(async, tail) : batch_fn<double, 32, SVM>(async, tail);
-    default: return fp32 ? batch_fn<float, 64, SVM>(async, tail) : batch_fn<double, 64, SVM>(async, tail);
+    case 8: return fp32 ? batch_fn<float, 8, SVM, WS>(async, tail) : batch_fn<double, 8, SVM, WS>(async, tail);
+    case 16: return fp32 ? batch_fn<float, 16, SVM, WS>(async, tail) : batch_fn<double, 16, SVM, WS>(async, tail);
+    case 32: return fp32 ? batch_fn<float, 32, SVM, WS>(async, tail) : batch_fn<double, 32, SVM, WS>(async, tail);
+    default: return fp32 ? batch_fn<float, 64, SVM, WS>(async, tail) : batch_fn<double, 64, SVM, WS>(async, tail);
   }
 }
 const void *batch_svm_kernel(bool fp32, int lanes, bool async, int64_t maxlen);
+const void *batch_ws_kernel(bool fp32, int lanes, int64_t maxlen);
 
 }  // namespace acdc

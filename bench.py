@@ -261,7 +261,7 @@ def batched(acdc, args, checks):
     persistent launch with asynchronous workers inside each instance, the
     spectral ESO beta per instance and working sets (screening: strong rule
     and gap-safe screening, capped by |g_j| rank, one cyclic order, Anderson
-    extrapolation), in FP32 + FP64 refresh and, the same kernel and
+    extrapolation, support Newton-CG steps), in FP32 + FP64 refresh and, the same kernel and
     settings, in FP64: the measured whole-workload ratio, and the ratio of
     the two runs' device time summed over the pairs both brought to the gap.
     Each pair runs until it meets the gap or its cap of max_epochs epochs.
@@ -278,11 +278,13 @@ def batched(acdc, args, checks):
     tuned = acdc.batch_tune(probs, step_mode=acdc.STEP_ASYNC, beta_mode=acdc.BETA_SPECTRAL)
     log(f"[bench] batched: autotuned lanes {tuned['lanes_per_coord']}, tau {tuned['tau']}, "
         f"queue chunk {tuned['queue_chunk']}")
+    newton = acdc.WS_NEWTON_ITERS if args.batch_newton_iters is None else args.batch_newton_iters
     out = {"workload": f"{len(ds)} lasso instances x {args.batch_path} lambdas",
            "nnz": int(nnz.sum()), "max_epochs_per_lambda": args.batch_max_epochs,
            "gap_tol": args.gap_target, "beta_mode": "spectral (working sets: omega rule)",
            "screening": "working sets: strong rule + gap-safe screening, capped by |g| rank, "
-                        "cyclic order, Anderson extrapolation", "autotuned": tuned}
+                        "cyclic order, Anderson extrapolation, support Newton-CG steps",
+           "newton_cg_iters": newton, "autotuned": tuned}
     order = np.argsort(nnz)
     sample = [int(order[int(q * (len(ds) - 1))]) for q in np.linspace(0.0, 1.0, 8)]
     conv, pair_us = {}, {}
@@ -291,7 +293,7 @@ def batched(acdc, args, checks):
                        max_epochs=args.batch_max_epochs, gap_tol=args.gap_target,
                        step_mode=acdc.STEP_ASYNC, lanes_per_coord=tuned["lanes_per_coord"],
                        tau=tuned["tau"], queue_chunk=tuned["queue_chunk"],
-                       beta_mode=acdc.BETA_SPECTRAL, screening=True)
+                       beta_mode=acdc.BETA_SPECTRAL, screening=True, ws_newton_iters=newton)
         t0 = time.perf_counter()
         ms = b.run()
         wall = time.perf_counter() - t0
@@ -631,6 +633,9 @@ def main():
     ap.add_argument("--batch-path", type=int, default=32)
     ap.add_argument("--batch-max-epochs", type=int, default=500,
                     help="cap on the updates of one (instance, lambda) pair, in epochs of n")
+    ap.add_argument("--batch-newton-iters", type=int, default=None,
+                    help="CG iterations of the working sets' support Newton steps "
+                         "(0: none; default: the library's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "

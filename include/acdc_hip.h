@@ -315,7 +315,21 @@ typedef struct acdc_batch_settings {
                               converged).  0 = every coordinate throughout.
                               The optimum is the same; the coordinate
                               sequence is not. */
+  int32_t ws_newton_iters; /* screened Lasso instances: at most this many
+                              conjugate-gradient iterations for a Newton
+                              step on the support of x at each restricted
+                              refresh of a working set (the quadratic of the
+                              support's sign orthant; the step is projected
+                              onto that orthant and kept only when the FP64
+                              objective falls); 0 = no such steps, at most
+                              ACDC_WS_NEWTON_MAX_ITERS.  It makes the
+                              near-interpolating pairs of a lambda path
+                              (support near m) converge in a few hundred
+                              epochs instead of tens of thousands.  No
+                              counterpart in the reference. */
 } acdc_batch_settings;
+
+#define ACDC_WS_NEWTON_MAX_ITERS 10000
 
 typedef struct acdc_batch_result {
   double lambda, primal, dual, rel_gap, nnz_x;
@@ -327,6 +341,8 @@ typedef struct acdc_batch_result {
                        held a NaN or an infinity (data or iterate), so it was
                        abandoned at this lambda and the rest of its path;
                        the other instances are unaffected */
+  int32_t newton_steps; /* support Newton steps kept for this lambda (screened
+                           Lasso instances with ws_newton_iters > 0) */
 } acdc_batch_result;
 #define ACDC_BATCH_NONFINITE 1
 

@@ -418,16 +418,17 @@ def test_batch_spectral_beta_svm(acdc, svm_batch):
     b.close()
 
 
-@pytest.mark.parametrize("prec,beta", [("fp32", "omega"), ("fp64", "omega"),
-                                       ("fp32", "spectral")])
-def test_batch_screening_reaches_oracle_gap(acdc, orc, small_batch, prec, beta):
+@pytest.mark.parametrize("prec,beta,newton", [("fp32", "omega", None), ("fp64", "omega", None),
+                                              ("fp32", "spectral", None), ("fp32", "spectral", 0)])
+def test_batch_screening_reaches_oracle_gap(acdc, orc, small_batch, prec, beta, newton):
     """Working sets (screening on: strong rule, gap-safe screening, a cap by
-    |g_j| rank, a cyclic sweep, Anderson extrapolation; under the spectral
-    rule W's ESO constant from the power method) for asynchronous Lasso
-    instances along a path down to 1e-3 lambda_max: every pair converges,
-    the oracle's evaluation of each instance's last x has relative gap
-    <= 1e-6 and equals the device's report, and the workers drew from
-    working sets smaller than n along the path."""
+    |g_j| rank, a cyclic sweep, Anderson extrapolation, support Newton steps
+    unless newton = 0; under the spectral rule W's ESO constant from the
+    power method) for asynchronous Lasso instances along a path down to
+    1e-3 lambda_max: every pair converges, the oracle's evaluation of each
+    instance's last x has relative gap <= 1e-6 and equals the device's
+    report, and the workers drew from working sets smaller than n along the
+    path."""
     ds = small_batch
     lams = lambdas_of(acdc, ds, 8, 1e-3)
     probs = [d.problem() for d in ds]
@@ -437,7 +438,7 @@ def test_batch_screening_reaches_oracle_gap(acdc, orc, small_batch, prec, beta):
     # the three wide instances (measured with the oracle on like data)
     b = acdc.Batch(probs, lams, seed=11, precision=precision, refresh_period=5, max_epochs=40000,
                    gap_tol=1e-6, use_graph=True, step_mode=acdc.STEP_ASYNC, screening=True,
-                   beta_mode=beta_mode)
+                   beta_mode=beta_mode, ws_newton_iters=newton)
     b.run()
     res = b.results()
     print("epochs", res["epochs"].tolist(), "ws", res["ws_size"].tolist(), flush=True)
@@ -450,6 +451,41 @@ def test_batch_screening_reaches_oracle_gap(acdc, orc, small_batch, prec, beta):
     ns = np.array([d.n for d in ds])[:, None]
     assert (res["ws_size"][:, 1:] < ns).mean() > 0.5, res["ws_size"]
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batch_newton_steps_on_near_interpolating_pairs(acdc, orc, prec):
+    """Wide instances (n = 8-20 m) down to 1e-3 lambda_max, where the support
+    of the small-lambda solutions nears m: with the support Newton steps
+    every pair reaches gap 1e-6 (the oracle's evaluation of each last x
+    agrees) within a cap of 500 epochs per lambda, and the path takes at
+    most a third of the epochs the working sets without them take (capped
+    at 2,000 per lambda)."""
+    ds = [acdc.gen_lasso(m, n, nnz_law=acdc.NNZ_ZIPF_TRUNC, nnz_lo=3, nnz_hi=100, zipf_s=1.1,
+                         xstar_nnz=max(1, n // 100), xstar_normal=False, noise=0.01,
+                         lambda_frac=0.1, seed=700 + i)
+          for i, (m, n) in enumerate([(400, 8000), (900, 9000), (600, 12000), (1000, 20000)])]
+    lams = lambdas_of(acdc, ds, 12, 1e-3)
+    probs = [d.problem() for d in ds]
+    precision = acdc.PREC_FP32_REFRESH if prec == "fp32" else acdc.PREC_FP64
+    out = {}
+    for it, cap in ((200, 500), (0, 2000)):
+        b = acdc.Batch(probs, lams, seed=13, precision=precision, refresh_period=5,
+                       max_epochs=cap, gap_tol=1e-6, use_graph=True, step_mode=acdc.STEP_ASYNC,
+                       screening=True, beta_mode=acdc.BETA_SPECTRAL, ws_newton_iters=it)
+        b.run()
+        res = b.results()
+        print("newton", it, "epochs", res["epochs"].tolist(), "nnz", res["nnz_x"][:, -1].tolist(),
+              "steps kept", res["newton_steps"].tolist(), flush=True)
+        if it:
+            assert res["converged"].all(), res["rel_gap"]
+            for i, d in enumerate(ds):
+                e = orc.CD(0, d.m, d.n, d.ptr, d.idx, d.vals, d.rhs, lams[i, -1]).eval(b.x(i))
+                assert e["rel_gap"] <= 1e-6, (i, e["rel_gap"])
+                assert res[i, -1]["primal"] == pytest.approx(e["primal"], rel=1e-9)
+        out[it] = int(res["epochs"].sum())
+        b.close()
+    assert out[200] * 3 <= out[0], out
 
 
 @pytest.mark.parametrize("mode", ["sync", "async", "async_screened"])
