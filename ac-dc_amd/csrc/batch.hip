@@ -255,6 +255,8 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
   if (hipMemset(dent + tot_nnz * esz, 0, esz) != hipSuccess)  // the pad entry: row 0, value 0
     return bail(bfail(ACDC_ERR_HIP, "acdc_batch_create: pad entry"));
   if ((st = balloc(b, &drent, tot_nnz)) != ACDC_OK) return bail(st);
+  RefEntry *dsent = nullptr;  // the support entries of the working sets' Newton steps
+  if (ws && (st = balloc(b, &dsent, tot_nnz)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &db, tot_m)) != ACDC_OK) return bail(st);
   if ((st = balloc(b, &dr64, tot_m)) != ACDC_OK) return bail(st);
   int64_t glob_m = 0;
@@ -283,6 +285,7 @@ extern "C" acdc_status acdc_batch_create(const acdc_problem *const *problems, in
     I.anx = ws ? dan + on * kWsScratchSlots : nullptr;
     I.ent = dent + onnz * esz;
     I.rent = drent + onnz;
+    I.sent = ws ? dsent + onnz : nullptr;
     I.nnz = (int32_t)p->nnz;
     I.loss = set->loss;  // the LossId values are the ABI's
     I.scale = 0;

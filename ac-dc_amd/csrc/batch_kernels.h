@@ -67,6 +67,8 @@ struct BatchInstance {
   int32_t wperm;         // nonzero on a working set, whose records are stored in
                          // a keyed random order: the workers sweep them
                          // cyclically (see batch_async_updates)
+  RefEntry *sent;        // working sets: nnz entries of scratch (the support's
+                         // entries of a Newton step), or null
   double *anx;           // working sets: kWsScratchSlots x n doubles: the
                          // kAaSnapshots snapshots of x of the Anderson
                          // extrapolation, then the vectors of the support
@@ -1123,11 +1125,16 @@ __device__ __forceinline__ bool batch_aa_extrapolate(const BatchInstance &J, int
 // The workers then continue from x' (coordinates enter and leave S) and the
 // next restricted refresh steps again.  FP64 throughout: q = A_S p lives in
 // the storage of r64 (LDS when its m doubles fit, the instance's global r64
-// otherwise); S is compacted into the six slots of I.anx after the
-// snapshots (x_S, d, the CG residual, p, and 16 bytes of column per
-// coordinate).  Returns whether x' was written to the records; the
-// correction residual dr is zero on return either way.
-constexpr int kWsScratchSlots = kAaSnapshots + 6;  // doubles of I.anx per coordinate
+// otherwise), and p beside it in LDS when both fit; S is compacted into
+// the seven slots of I.anx after the snapshots (x_S, d, the CG residual, p,
+// 16 bytes of column per coordinate, and h = A_S^T q), and the entries of
+// its columns are copied, contiguous and
+// tagged with the compacted column, into I.sent, so that the products with
+// A_S stream them: four entries in flight per thread for A_S p, segmented
+// sums over wave-contiguous ranges for A_S^T q.  Returns whether x' was
+// written to the records; the correction residual dr is zero on return
+// either way.
+constexpr int kWsScratchSlots = kAaSnapshots + 7;  // doubles of I.anx per coordinate
 constexpr int kNewtonSearch = 8;                   // step lengths the search tries
 constexpr double kNewtonDamping = 1e-4;            // mu over the mean ||a_j||^2
 
@@ -1144,16 +1151,41 @@ __device__ __forceinline__ double block_sum_all(double v, double *sh) {
   return t;
 }
 
-// a support column of a Newton step: its slot in W and its entries
+// a support column of a Newton step: its slot in W, its entries in the
+// instance's entries, and where its copy starts in the support's entries
 struct alignas(16) NewtonCol {
-  int32_t u, p0, len, pad;
+  int32_t u, p0, len, at;
 };
 
+// exclusive prefix sum over the block of one int per thread (thread order);
+// *total receives the block's sum.  wsum: kBatchWaves ints of scratch.
+__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total) {
+  const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int s = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(s, o, 64);
+    if (ln >= o) s += t;
+  }
+  if (ln == 63) wsum[wv] = s;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kBatchWaves; ++k) {
+    const int c = wsum[k];
+    off += k < wv ? c : 0;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + s - v;
+}
 
+// lds_doubles: the doubles the dynamic LDS holds
 template <typename T, bool LDSR, bool RL>
 __device__ __forceinline__ bool batch_ws_newton(const BatchInstance &J, int32_t nw, int64_t stride,
-                                             double lambda, double p0, int iters, T *lds,
-                                             double *sh) {
+                                                double lambda, double p0, int iters, T *lds,
+                                                int lds_doubles, double *sh) {
   const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63;
   double *const q = RL ? reinterpret_cast<double *>(lds) : J.r64;
   double *const xs = J.anx + (int64_t)kAaSnapshots * stride;  // x on S, compacted
@@ -1161,6 +1193,8 @@ __device__ __forceinline__ bool batch_ws_newton(const BatchInstance &J, int32_t 
   double *const rv = dv + stride;                             // CG residual
   double *const pv = rv + stride;                             // CG direction
   NewtonCol *const cols = reinterpret_cast<NewtonCol *>(pv + stride);
+  double *const hv = pv + 3 * stride;  // h = A_S^T q
+  RefEntry *const se = J.sent;         // the support's entries, column k's with col = k
   auto qget = [&](int32_t k) -> double { return RL ? q[k] : gld_agent(q + k); };
   auto qset = [&](int32_t k, double v) {
     if (RL)
@@ -1174,78 +1208,174 @@ __device__ __forceinline__ bool batch_ws_newton(const BatchInstance &J, int32_t 
     else
       gadd_block(q + k, v);
   };
-  // S compacted in W order, with res = p = -(g_S + lambda s) and d = 0
   int *wc = reinterpret_cast<int *>(sh);
-  int32_t ns = 0;
+  // S compacted in W order (with the offsets of the columns' entry copies),
+  // res = p = -(g_S + lambda s), d = 0
+  int32_t ns = 0, ne = 0;
   double rr = 0.0;
   for (int32_t u0 = 0; u0 < nw; u0 += kBatchBlock) {
     const int32_t u = u0 + tid;
     const double x = u < nw ? gld_agent(&J.cs[u].x) : 0.0;  // workers add to x atomically
     const bool keep = x != 0.0;
-    const uint64_t bal = __ballot(keep);
-    if (ln == 0) wc[wv] = __popcll(bal);
-    __syncthreads();
-    int off = 0, tot = 0;
-#pragma unroll
-    for (int k = 0; k < kBatchWaves; ++k) {
-      const int c = wc[k];
-      off += k < wv ? c : 0;
-      tot += c;
-    }
+    CoordRec r{};
+    if (keep) r = gld(J.cs + u);
+    int cnt = 0, tot = 0;
+    const int k = ns + block_excl_scan(keep ? 1 : 0, wc, &cnt);
+    const int at = ne + block_excl_scan(keep ? r.len : 0, wc, &tot);
     if (keep) {
-      const int32_t k = ns + off + __popcll(bal & ((1ull << ln) - 1ull));
-      const CoordRec r = gld(J.cs + u);
       const double g = -(r.gref + lambda * sign_of(x));
       *gp(xs + k) = x;
       *gp(dv + k) = 0.0;
       *gp(rv + k) = g;
       *gp(pv + k) = g;
-      cols[k] = NewtonCol{u, r.p0, r.len, 0};
+      cols[k] = NewtonCol{u, r.p0, r.len, at};
       rr += g * g;
     }
-    ns += tot;
-    __syncthreads();  // wc is rewritten by the next chunk
+    ns += cnt;
+    ne += tot;
+  }
+  __syncthreads();
+  // p in LDS after q when both fit
+  const bool pl = RL && (int64_t)J.m + ns <= lds_doubles;
+  double *const pls = reinterpret_cast<double *>(lds) + J.m;
+  auto pget = [&](int32_t k) -> double { return pl ? pls[k] : gld_agent(pv + k); };
+  if (pl)
+    for (int32_t k = tid; k < ns; k += kBatchBlock) pls[k] = gld_agent(pv + k);
+  // the copy of the support's entries (16 lanes per column)
+  {
+    constexpr int G = 16;
+    const int grp = tid / G, lane = tid & (G - 1);
+    for (int32_t k = grp; k < ns; k += kBatchBlock / G) {
+      const NewtonCol c = gld(cols + k);
+      for (int32_t i = lane; i < c.len; i += G) {
+        RefEntry e = gld(J.rent + c.p0 + i);
+        e.col = k;
+        se[c.at + i] = e;
+      }
+    }
   }
   rr = block_sum_all(rr, sh);  // its barriers also publish the stores above
   const double mu = kNewtonDamping * J.colsq;
-  // A_S passes: each G-lane group takes U columns at a time, their records
-  // and first G entries loaded before any is used (a column's dependent
-  // loads would otherwise be the pass's whole cost)
-  constexpr int G = 16, NG = kBatchBlock / G, U = 4;
-  const int grp = tid / G, lane = tid & (G - 1);
-  // q += c_k a_k, c_k = coef(k) (0: skipped); returns the lane's share of
+  // q += c_k a_k over the support's entries, c_k = coef(k) (0: skipped),
+  // four entries per thread in flight; returns the thread's share of
   // sum_k c_k^2 ||a_k||^2
   auto scatter = [&](auto coef) {
     double sq = 0.0;
-    for (int32_t k0 = grp; k0 < ns; k0 += NG * U) {
-      NewtonCol cl[U];
-      double c[U];
-      RefEntry e[U];
+    RefEntry nx[4];  // the next round's entries, in flight while this round is applied
+    auto load = [&](int32_t i0) {
 #pragma unroll
-      for (int j = 0; j < U; ++j) {
-        const int32_t k = k0 + j * NG;
-        cl[j] = k < ns ? gld(cols + k) : NewtonCol{0, 0, 0, 0};
-        c[j] = k < ns ? coef(k) : 0.0;
+      for (int j = 0; j < 4; ++j) {
+        const int32_t i = i0 + j * kBatchBlock;
+        nx[j] = i < ne ? gld(se + i) : RefEntry{0, 0, 0.0};
       }
+    };
+    load(tid);
+    for (int32_t i0 = tid; i0 < ne; i0 += 4 * kBatchBlock) {
+      RefEntry e[4];
+      double c[4];
 #pragma unroll
-      for (int j = 0; j < U; ++j) e[j] = lane < cl[j].len ? gld(J.rent + cl[j].p0 + lane) : RefEntry{};
+      for (int j = 0; j < 4; ++j) e[j] = nx[j];
+      load(i0 + 4 * kBatchBlock);
 #pragma unroll
-      for (int j = 0; j < U; ++j) {
-        if (c[j] == 0.0) continue;  // uniform in the group
-        if (lane < cl[j].len) {
+      for (int j = 0; j < 4; ++j) c[j] = i0 + j * kBatchBlock < ne ? coef(e[j].col) : 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (c[j] != 0.0) {
           const double v = e[j].val * c[j];
           qadd(e[j].row, v);
           sq += v * v;
         }
-        for (int32_t k = lane + G; k < cl[j].len; k += G) {
-          const RefEntry f = gld(J.rent + cl[j].p0 + k);
-          const double v = f.val * c[j];
-          qadd(f.row, v);
-          sq += v * v;
+    }
+    return sq;
+  };
+  // h_k = <a_k, q> for every support column, emit(k, h_k) once per column:
+  // wave w streams the support's entries [ne w/16, ne (w+1)/16), 64 per step,
+  // sums runs of equal column by segmented scans and emits each run it
+  // completes; the wave's first run (it may have begun in the previous
+  // range) and the run open at its end are joined by thread 0 afterwards
+  auto seg_gather = [&](auto emit) {
+    double *slot = sh + 4 * wv;  // head column, head sum, tail column, tail sum
+    __syncthreads();             // sh may still be read by a block sum
+    if (ln == 0) slot[0] = -1.0;
+    const int32_t e_lo = (int32_t)((int64_t)ne * wv / kBatchWaves);
+    const int32_t e_hi = (int32_t)((int64_t)ne * (wv + 1) / kBatchWaves);
+    int first_col = -1, open_col = -1;
+    double open_sum = 0.0;
+    auto put = [&](int c, double v) {
+      if (c == first_col) {
+        slot[0] = (double)c;
+        slot[1] = v;
+      } else {
+        emit(c, v);
+      }
+    };
+    constexpr int UG = 4;  // steps of 64 entries per round
+    RefEntry nxt[UG];
+    auto load = [&](int32_t base) {
+#pragma unroll
+      for (int j = 0; j < UG; ++j) nxt[j] = gld(se + min(base + j * 64 + ln, e_hi - 1));
+    };
+    if (e_lo < e_hi) {
+      load(e_lo);
+      first_col = __builtin_amdgcn_readfirstlane(nxt[0].col);
+      for (int32_t base = e_lo; base < e_hi; base += 64 * UG) {
+        RefEntry cur[UG];
+#pragma unroll
+        for (int j = 0; j < UG; ++j) cur[j] = nxt[j];
+        load(base + 64 * UG);
+        double qv[UG];
+#pragma unroll
+        for (int j = 0; j < UG; ++j) qv[j] = qget(cur[j].row);
+#pragma unroll
+        for (int j = 0; j < UG; ++j) {
+          const int32_t cb = base + j * 64;
+          if (cb >= e_hi) break;  // wave-uniform
+          const bool in = cb + ln < e_hi;
+          const int key = in ? cur[j].col : 0x7fffffff;
+          const SegSum r = seg_scan_sum(in ? cur[j].val * qv[j] : 0.0, key);
+          double v = r.sum;
+          const int k0 = __builtin_amdgcn_readfirstlane(key);
+          if (open_col >= 0) {
+            if (k0 == open_col) {
+              if (key == open_col) v += open_sum;
+            } else if (ln == 0) {
+              put(open_col, open_sum);  // the open run ended with the last step
+            }
+          }
+          const int lv = min(63, e_hi - 1 - cb);  // last lane in range
+          const uint64_t heads = __ballot(r.head);
+          if (ln < lv && ((heads >> (ln + 1)) & 1)) put(key, v);
+          open_col = __builtin_amdgcn_readlane(key, lv);
+          const long long vb = __double_as_longlong(v);
+          const int lo = __builtin_amdgcn_readlane((int)vb, lv);
+          const int hi = __builtin_amdgcn_readlane((int)(vb >> 32), lv);
+          open_sum = __longlong_as_double(
+              (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
         }
       }
     }
-    return sq;
+    if (ln == 0) {
+      slot[2] = (double)open_col;
+      slot[3] = open_sum;
+    }
+    __syncthreads();
+    if (tid == 0) {  // join the partial runs, in entry order
+      int c = -1;
+      double v = 0.0;
+      for (int k = 0; k < 2 * kBatchWaves; ++k) {
+        const int kc = (int)sh[2 * k];
+        if (kc < 0) continue;
+        if (kc == c) {
+          v += sh[2 * k + 1];
+          continue;
+        }
+        if (c >= 0) emit(c, v);
+        c = kc;
+        v = sh[2 * k + 1];
+      }
+      if (c >= 0) emit(c, v);
+    }
+    __syncthreads();
   };
   double p2 = rr;  // ||p||^2
   const double rr0 = rr;
@@ -1254,7 +1384,7 @@ __device__ __forceinline__ bool batch_ws_newton(const BatchInstance &J, int32_t 
     for (int32_t k = tid; k < J.m; k += kBatchBlock) qset(k, 0.0);
     __syncthreads();
     // pp = sum_k p_k^2 ||a_k||^2, the curvature's scale
-    double pp = scatter([&](int32_t k) { return gld_agent(pv + k); });
+    double pp = scatter(pget);
     __syncthreads();
     double qq = 0.0;
     for (int32_t k = tid; k < J.m; k += kBatchBlock) {
@@ -1266,46 +1396,25 @@ __device__ __forceinline__ bool batch_ws_newton(const BatchInstance &J, int32_t 
     const double curv = qq + mu * p2;  // p^T (H + mu I) p
     if (!(curv > 1e-10 * pp)) break;
     const double al = rr / curv;
-    // h_k = <a_k, q> + mu p_k: d += al p, res -= al h
+    // h = A_S^T q; then d += al p, res -= al (h + mu p)
+    seg_gather([&](int32_t k, double h) { *gp(hv + k) = h; });
     double rn = 0.0;
-    for (int32_t k0 = grp; k0 < ns; k0 += NG * U) {
-      NewtonCol cl[U];
-      RefEntry e[U];
-      double h[U];
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        const int32_t k = k0 + j * NG;
-        cl[j] = k < ns ? gld(cols + k) : NewtonCol{0, 0, 0, 0};
-      }
-#pragma unroll
-      for (int j = 0; j < U; ++j) e[j] = lane < cl[j].len ? gld(J.rent + cl[j].p0 + lane) : RefEntry{};
-#pragma unroll
-      for (int j = 0; j < U; ++j) h[j] = lane < cl[j].len ? e[j].val * qget(e[j].row) : 0.0;
-#pragma unroll
-      for (int j = 0; j < U; ++j)
-        for (int32_t k = lane + G; k < cl[j].len; k += G) {
-          const RefEntry f = gld(J.rent + cl[j].p0 + k);
-          h[j] += f.val * qget(f.row);
-        }
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        const int32_t k = k0 + j * NG;
-        const double hs = group_sum<G>(h[j]);
-        if (lane == 0 && k < ns) {
-          const double pk = gld_agent(pv + k);
-          *gp(dv + k) = gld_agent(dv + k) + al * pk;
-          const double r = gld_agent(rv + k) - al * (hs + mu * pk);
-          *gp(rv + k) = r;
-          rn += r * r;
-        }
-      }
+    for (int32_t k = tid; k < ns; k += kBatchBlock) {
+      const double pk = pget(k);
+      *gp(dv + k) = gld_agent(dv + k) + al * pk;
+      const double r = gld_agent(rv + k) - al * (gld_agent(hv + k) + mu * pk);
+      *gp(rv + k) = r;
+      rn += r * r;
     }
     rn = block_sum_all(rn, sh);
     const double be = rn / rr;
     double pn2 = 0.0;
     for (int32_t k = tid; k < ns; k += kBatchBlock) {
-      const double pk = gld_agent(rv + k) + be * gld_agent(pv + k);
-      *gp(pv + k) = pk;
+      const double pk = gld_agent(rv + k) + be * pget(k);
+      if (pl)
+        pls[k] = pk;
+      else
+        *gp(pv + k) = pk;
       pn2 += pk * pk;
     }
     p2 = block_sum_all(pn2, sh);  // (its barriers publish p)
@@ -1319,8 +1428,8 @@ __device__ __forceinline__ bool batch_ws_newton(const BatchInstance &J, int32_t 
     return sign_of(xn) == sign_of(x) ? xn : 0.0;
   };
   bool acc = false;
-  for (int s = 0; it > 0 && s < kNewtonSearch && !acc; ++s) {
-    const double t = ldexp(1.0, -s);
+  for (int st = 0; it > 0 && st < kNewtonSearch && !acc; ++st) {
+    const double t = ldexp(1.0, -st);
     // q = A x' - b
     for (int32_t r = tid; r < J.m; r += kBatchBlock) qset(r, -gld(J.b + r));
     __syncthreads();
@@ -1778,11 +1887,12 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           // a support Newton step from the restricted refresh just run
           auto newton = [&](const BatchInstance &Jw, int32_t n_w) {
             const double p0 = s_res.primal;
-            if (in_lds ? I.m <= a.lds_doubles : false)
-              return batch_ws_newton<T, true, true>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, sh);
+            const int ld = a.lds_doubles;
+            if (in_lds ? I.m <= ld : false)
+              return batch_ws_newton<T, true, true>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, ld, sh);
             if (in_lds)
-              return batch_ws_newton<T, true, false>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, sh);
-            return batch_ws_newton<T, false, false>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, sh);
+              return batch_ws_newton<T, true, false>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, ld, sh);
+            return batch_ws_newton<T, false, false>(Jw, n_w, I.n, lambda, p0, a.ws_newton, lds, ld, sh);
           };
           auto refresh_ws = [&]() {
             if (in_lds ? I.m <= a.lds_doubles : false)
