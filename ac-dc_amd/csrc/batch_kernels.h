@@ -626,6 +626,7 @@ __device__ __forceinline__ void batch_refresh(const BatchInstance *Ip, double la
 // the top: the candidates of the highest bins that fit the cap enter, and
 // the violators always do.
 constexpr int kWsBins = 128;
+constexpr int32_t kWsWorkerSpan = 8;  // coordinates of W per asynchronous worker, at least
 constexpr int32_t kWsMin = 256;
 static_assert(kWsBins + kBatchWaves + 1 <= 2 * kRefreshStats * kBatchWaves,
               "the histogram and the waves' counts fit the refresh scratch");
@@ -1122,8 +1123,11 @@ __device__ __forceinline__ bool batch_aa_extrapolate(const BatchInstance &J, int
 // 1, 1/2, ..., 1/128 for which x' = P_s(x_S + t d) -- every coordinate that
 // would change sign set to 0, the orthant projection of OWL-QN (Andrew and
 // Gao, ICML 2007) -- has an FP64 objective below the refresh's primal p0.
-// The workers then continue from x' (coordinates enter and leave S) and the
-// next restricted refresh steps again.  FP64 throughout: q = A_S p lives in
+// The workers then continue from x' (coordinates enter and leave S).  A step
+// is only taken where the workers stall -- W's gap fell by less than half
+// (kNewtonStall) since the previous restricted refresh -- so that
+// well-conditioned pairs, which coordinate descent finishes in a few
+// epochs, are left to it.  FP64 throughout: q = A_S p lives in
 // the storage of r64 (LDS when its m doubles fit, the instance's global r64
 // otherwise), and p beside it in LDS when both fit; S is compacted into
 // the seven slots of I.anx after the snapshots (x_S, d, the CG residual, p,
@@ -1137,6 +1141,7 @@ __device__ __forceinline__ bool batch_aa_extrapolate(const BatchInstance &J, int
 constexpr int kWsScratchSlots = kAaSnapshots + 7;  // doubles of I.anx per coordinate
 constexpr int kNewtonSearch = 8;                   // step lengths the search tries
 constexpr double kNewtonDamping = 1e-4;            // mu over the mean ||a_j||^2
+constexpr double kNewtonStall = 0.5;  // W's gap ratio between refreshes that calls a step
 
 // the block's sum of one double per thread, in a fixed order, returned to
 // every thread (so that all of them take the same branches on it)
@@ -1866,7 +1871,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           BatchInstance J = I;
           J.cs = I.ws;
           J.n = nw;
-          J.tau = min(I.tau, nw);
+          // at most one worker per kWsWorkerSpan coordinates of W: on a small
+          // W, many more workers update the same coordinates from stale
+          // residuals (measured: pairs with |W| ~ 250 and 128 workers stalled
+          // at the cap in FP64)
+          J.tau = max(1, min(I.tau, nw / kWsWorkerSpan));
           J.wperm = 1;
           // the ESO constant of W: omega_W, or with the spectral rule the
           // power method's estimate on W (10% margin: it approaches
@@ -1913,6 +1922,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
           // refreshes (gap check, FP64 anchor) every kWsRefreshPeriods of
           // them and around each extrapolation
           int aa_n = 0, aa_h = 0;  // snapshots held, next slot
+          double gap_prev = screen_gap;  // W's gap at the last restricted refresh
           int since = 0;           // periods since the last restricted refresh
           while (done < cap) {
             const uint64_t per =
@@ -1926,9 +1936,14 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_async(BatchArgs a) {
               refresh_ws();
               since = 0;
               if (w_done()) break;
-              if (I.anx && a.ws_newton > 0 && newton(J, nw)) {
+              // a Newton step only where the workers stall: W's gap fell by
+              // less than half since the last restricted refresh
+              const bool stalled = gap_w > kNewtonStall * gap_prev;
+              gap_prev = gap_w;
+              if (stalled && I.anx && a.ws_newton > 0 && newton(J, nw)) {
                 if (threadIdx.x == 0) ++s_res.newton;
                 refresh_ws();
+                gap_prev = gap_w;
                 if (w_done()) break;
                 aa_n = 0;  // the snapshots no longer lead to x: a new history
               }
