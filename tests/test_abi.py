@@ -96,6 +96,9 @@ def test_batch_argument_errors(acdc):
         acdc.Batch([csc], lam, lanes_per_coord=12)
     with pytest.raises(acdc.AcdcError, match="beta_mode"):
         acdc.Batch([csc], lam, beta_mode=acdc.BETA_FORCED)
+    for it in (-1, 10001):
+        with pytest.raises(acdc.AcdcError, match="ws_newton_iters"):
+            acdc.Batch([csc], lam, screening=True, ws_newton_iters=it)
     with pytest.raises(acdc.AcdcError, match="SVM problems must be sample rows"):
         acdc.batch_tune([csc], loss=acdc.LOSS_SVM_DUAL_HINGE)
 
