@@ -1104,39 +1104,36 @@ __device__ __forceinline__ bool batch_aa_extrapolate(const BatchInstance &J, int
 
 // ------------------------------------------------------ support Newton steps
 // Near interpolation (support close to m, or above it on the way) the
-// least-squares problem on the support S is ill-conditioned: a coordinate
-// epoch contracts the error by about 1 - 1/kappa of H = A_S^T A_S, so a pair
-// needs tens of thousands of epochs, where conjugate gradients need about
+// least-squares problem on the support S is ill-conditioned: a coordinate epoch
+// contracts the error by about 1 - 1/kappa of H = A_S^T A_S, so a pair needs
+// tens of thousands of epochs, where conjugate gradients need about
 // sqrt(kappa) products with H.  At a restricted refresh (r64 = A x - b exact,
 // g_u = <a_u, r64> in the records), with s the signs of x on S, the Lasso
 // objective on the orthant of s is the quadratic
 //   1/2 ||A_S x_S - b||^2 + lambda s^T x_S,
-// whose Newton direction d solves H d = -(g_S + lambda s).  When |S| > m
-// (the iterate's support passes m on the way to a solution with at most m
-// nonzeros) H is singular and the right-hand side has a part in its null
-// space, along which CG steps grow without bound; the damped system
-// (H + mu I) d = -(g_S + lambda s), mu = kNewtonDamping times the mean
-// ||a_j||^2 of the instance's columns (the Levenberg-Marquardt damping),
-// bounds them.  At most
-// `iters` CG iterations from d = 0 approximate d.  The projected search then
-// takes the first t in
-// 1, 1/2, ..., 1/128 for which x' = P_s(x_S + t d) -- every coordinate that
-// would change sign set to 0, the orthant projection of OWL-QN (Andrew and
-// Gao, ICML 2007) -- has an FP64 objective below the refresh's primal p0.
-// The workers then continue from x' (coordinates enter and leave S).  A step
-// is only taken where the workers stall -- W's gap fell by less than half
-// (kNewtonStall) since the previous restricted refresh -- so that
-// well-conditioned pairs, which coordinate descent finishes in a few
-// epochs, are left to it.  FP64 throughout: q = A_S p lives in
-// the storage of r64 (LDS when its m doubles fit, the instance's global r64
-// otherwise), and p beside it in LDS when both fit; S is compacted into
-// the seven slots of I.anx after the snapshots (x_S, d, the CG residual, p,
-// 16 bytes of column per coordinate, and h = A_S^T q), and the entries of
-// its columns are copied, contiguous and
-// tagged with the compacted column, into I.sent, so that the products with
-// A_S stream them: four entries in flight per thread for A_S p, segmented
-// sums over wave-contiguous ranges for A_S^T q.  Returns whether x' was
-// written to the records; the correction residual dr is zero on return
+// whose Newton direction d solves H d = -(g_S + lambda s).  When |S| > m (the
+// iterate's support passes m on the way to a solution with at most m nonzeros)
+// H is singular and the right-hand side has a part in its null space, along
+// which CG steps grow without bound; the damped system (H + mu I) d = -(g_S +
+// lambda s), mu = kNewtonDamping times the mean ||a_j||^2 of the instance's
+// columns (the Levenberg-Marquardt damping), bounds them.  At most `iters` CG
+// iterations from d = 0 approximate d.  The projected search then takes the
+// first t in 1, 1/2, ..., 1/128 for which x' = P_s(x_S + t d) -- every
+// coordinate that would change sign set to 0, the orthant projection of OWL-QN
+// (Andrew and Gao, ICML 2007) -- has an FP64 objective below the refresh's
+// primal p0. The workers then continue from x' (coordinates enter and leave
+// S).  A step is only taken where the workers stall -- W's gap fell by less
+// than half (kNewtonStall) since the previous restricted refresh -- so that
+// well-conditioned pairs, which coordinate descent finishes in a few epochs,
+// are left to it.  FP64 throughout: q = A_S p lives in the storage of r64 (LDS
+// when its m doubles fit, the instance's global r64 otherwise), and p beside
+// it in LDS when both fit; S is compacted into the seven slots of I.anx after
+// the snapshots (x_S, d, the CG residual, p, 16 bytes of column per
+// coordinate, and h = A_S^T q), and the entries of its columns are copied,
+// contiguous and tagged with the compacted column, into I.sent, so that the
+// products with A_S stream them: four entries in flight per thread for A_S p,
+// segmented sums over wave-contiguous ranges for A_S^T q.  Returns whether x'
+// was written to the records; the correction residual dr is zero on return
 // either way.
 constexpr int kWsScratchSlots = kAaSnapshots + 7;  // doubles of I.anx per coordinate
 constexpr int kNewtonSearch = 8;                   // step lengths the search tries
